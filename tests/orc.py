@@ -1,0 +1,112 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle64.so, FP64) and of its
+FP32 build (oracle/liboracle32.so).  Test infrastructure: used only as the
+checker by tests/, smoke() and bench.py's CPU baseline."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+_libs = {}
+
+
+def lib(bits=64):
+    if bits not in _libs:
+        path = os.path.join(ORACLE_DIR, "liboracle%d.so" % bits)
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: %s (run make -C oracle)" % path)
+        _libs[bits] = C.CDLL(path)
+    return _libs[bits]
+
+
+def _dt(bits):
+    return np.float64 if bits == 64 else np.float32
+
+
+def _ptr(a, bits):
+    t = C.c_double if bits == 64 else C.c_float
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def fft1d(x, inverse=False):
+    x = np.ascontiguousarray(x, np.complex128)
+    out = np.zeros_like(x)
+    lib(64).orc_fft1d(C.c_int(x.size), _ptr(x.view(np.float64), 64),
+                      _ptr(out.view(np.float64), 64), C.c_int(1 if inverse else 0))
+    return out
+
+
+def fft2d(a, inverse=False):
+    a = np.ascontiguousarray(a, np.complex128).copy()
+    nxx, nzz = a.shape
+    lib(64).orc_fft2d(C.c_int(nxx), C.c_int(nzz), _ptr(a.view(np.float64), 64),
+                      C.c_int(1 if inverse else 0))
+    return a
+
+
+FIELD_KEYS = ("vp", "rho", "bbx", "bbz", "gam", "lam", "vwd", "tao")
+C1D_KEYS = ("iax", "ibx", "iaz", "ibz", "hax", "hbx", "haz", "hbz",
+            "kxsmr", "kxsmi", "kxscr", "kxsci", "kzsmr", "kzsmi", "kzscr", "kzsci")
+K2_KEYS = ("k2d1", "k2d2", "k2d3", "k2da")
+
+
+class _Args:
+    """Converts a job dict (float32 arrays, the engine's inputs) to the
+    oracle's precision and keeps the buffers alive for the call."""
+
+    def __init__(self, job, bits):
+        dt = _dt(bits)
+        self.keep = []
+        PT = C.POINTER(C.c_double if bits == 64 else C.c_float)
+
+        def arr(keys):
+            ptrs = (PT * len(keys))()
+            for i, k in enumerate(keys):
+                if k in job and job[k] is not None:
+                    a = np.ascontiguousarray(job[k], dt).ravel()
+                    self.keep.append(a)
+                    ptrs[i] = _ptr(a, bits)
+            return ptrs
+
+        self.fields = arr(FIELD_KEYS)
+        self.c1d = arr(C1D_KEYS)
+        self.k2 = arr(K2_KEYS)
+        self.rik = np.ascontiguousarray(job["rik"], dt)
+        self.gxg = np.ascontiguousarray(job["gxg"], np.int32)
+        self.gzg = np.ascontiguousarray(job["gzg"], np.int32)
+
+
+def forward(job, bits=64, want_state=False):
+    """One-shot forward modeling on the CPU.  job: dict with kind (0 acoustic,
+    1 viscoacoustic), nxx, nzz, nt, dt and the float32 arrays of the job
+    (window fields, 1-D coefficients, k powers, rik, gxg, gzg, xsg, zsg)."""
+    L = lib(bits)
+    dt = _dt(bits)
+    a = _Args(job, bits)
+    nt, nr = int(job["nt"]), len(a.gxg)
+    N = int(job["nxx"]) * int(job["nzz"])
+    rec = np.zeros(nt * nr, dt)
+    p = np.zeros(N, dt)
+    vx = np.zeros(N, dt)
+    vz = np.zeros(N, dt)
+    real = C.c_double if bits == 64 else C.c_float
+    L.orc_forward(C.c_int(int(job["kind"])), C.c_int(int(job["nxx"])), C.c_int(int(job["nzz"])),
+                  C.c_int(nt), real(float(job["dt"])), a.fields, a.c1d, a.k2, _ptr(a.rik, bits),
+                  C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])), C.c_int(nr),
+                  a.gxg.ctypes.data_as(C.POINTER(C.c_int)),
+                  a.gzg.ctypes.data_as(C.POINTER(C.c_int)), _ptr(rec, bits), _ptr(p, bits),
+                  _ptr(vx, bits), _ptr(vz, bits))
+    rec = rec.reshape(nt, nr)
+    if want_state:
+        return rec, {"p": p, "vx": vx, "vz": vz}
+    return rec
+
+
+def set_threads(n, bits=64):
+    lib(bits).orc_set_threads(C.c_int(n))
+
+
+def num_threads(bits=64):
+    return int(lib(bits).orc_num_threads())
