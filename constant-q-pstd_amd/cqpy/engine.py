@@ -1,0 +1,182 @@
+"""ctypes bindings of libcq_hip.so (include/cq.h).
+
+The product path: every call goes to the HIP engine; if the library or a GPU
+is missing the calls raise instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .host import PKG_DIR
+
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+
+
+class JobDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("nxx", C.c_int), ("nzz", C.c_int), ("pml", C.c_int),
+        ("nt", C.c_int), ("dt", C.c_float),
+        ("vp", _F), ("rho", _F), ("bbx", _F), ("bbz", _F),
+        ("gam", _F), ("lam", _F), ("vwd", _F), ("tao", _F),
+        ("iax", _F), ("ibx", _F), ("hax", _F), ("hbx", _F),
+        ("iaz", _F), ("ibz", _F), ("haz", _F), ("hbz", _F),
+        ("kxsmr", _F), ("kxsmi", _F), ("kxscr", _F), ("kxsci", _F),
+        ("kzsmr", _F), ("kzsmi", _F), ("kzscr", _F), ("kzsci", _F),
+        ("dx", C.c_float), ("dz", C.c_float),
+        ("rik", _F), ("xsg", C.c_int), ("zsg", C.c_int),
+        ("nr", C.c_int), ("gxg", _I), ("gzg", _I),
+    ]
+
+
+SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_create",
+           "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",
+           "cq_batch_read_p", "cq_batch_time", "cq_batch_bytes", "cq_batch_destroy",
+           "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times"]
+
+_lib = None
+
+
+def lib_path():
+    return os.path.join(PKG_DIR, "libcq_hip.so")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libcq_hip.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.cq_last_error.restype = C.c_char_p
+        L.cq_version.restype = C.c_char_p
+        L.cq_batch_bytes.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+class CqError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc != 0:
+        raise CqError("cq error %d: %s" % (rc, lib().cq_last_error().decode()))
+
+
+_F32_KEYS = ("vp", "rho", "bbx", "bbz", "gam", "lam", "vwd", "tao", "iax", "ibx", "hax", "hbx",
+             "iaz", "ibz", "haz", "hbz", "kxsmr", "kxsmi", "kxscr", "kxsci", "kzsmr", "kzsmi",
+             "kzscr", "kzsci", "rik")
+
+
+def make_desc(job, keep):
+    """cq_job_desc from a job dict (see cqpy.synth.make_job); `keep` collects
+    the arrays that must stay alive until the descriptor is consumed."""
+    d = JobDesc()
+    d.kind, d.nxx, d.nzz, d.pml = int(job["kind"]), int(job["nxx"]), int(job["nzz"]), int(job["pml"])
+    d.nt, d.dt = int(job["nt"]), float(job["dt"])
+    d.dx, d.dz = float(job["dx"]), float(job["dz"])
+    for k in _F32_KEYS:
+        if k in job and job[k] is not None:
+            a = np.ascontiguousarray(job[k], np.float32)
+            keep.append(a)
+            setattr(d, k, a.ctypes.data_as(_F))
+    gx = np.ascontiguousarray(job["gxg"], np.int32)
+    gz = np.ascontiguousarray(job["gzg"], np.int32)
+    keep += [gx, gz]
+    d.xsg, d.zsg, d.nr = int(job["xsg"]), int(job["zsg"]), gx.size
+    d.gxg, d.gzg = gx.ctypes.data_as(_I), gz.ctypes.data_as(_I)
+    return d
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _chk(lib().cq_open(C.c_int(device), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cq_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def plain_forward(self, job):
+        keep = []
+        d = make_desc(job, keep)
+        rec = np.zeros(int(job["nt"]) * d.nr, np.float32)
+        _chk(lib().cq_plain_forward(self.h, C.byref(d), rec.ctypes.data_as(_F)))
+        return rec.reshape(int(job["nt"]), d.nr)
+
+    def batch(self, jobs):
+        return Batch(self, jobs)
+
+
+class Batch:
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        self.jobs = jobs
+        keep = []
+        arr = (JobDesc * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = make_desc(j, keep)
+        self.h = C.c_void_p()
+        _chk(lib().cq_batch_create(ctx.h, arr, C.c_int(len(jobs)), C.byref(self.h)))
+
+    def step(self, n):
+        _chk(lib().cq_batch_step(self.h, C.c_int(n)))
+
+    def sync(self):
+        _chk(lib().cq_batch_sync(self.h))
+
+    def run(self):
+        _chk(lib().cq_batch_run(self.h))
+
+    def time(self):
+        return int(lib().cq_batch_time(self.h))
+
+    def nbytes(self):
+        return int(lib().cq_batch_bytes(self.h))
+
+    def rec(self, i):
+        j = self.jobs[i]
+        nr = len(j["gxg"])
+        out = np.zeros(int(j["nt"]) * nr, np.float32)
+        _chk(lib().cq_batch_read_rec(self.h, C.c_int(i), out.ctypes.data_as(_F)))
+        return out.reshape(int(j["nt"]), nr)
+
+    def p(self, i):
+        j = self.jobs[i]
+        out = np.zeros(int(j["nxx"]) * int(j["nzz"]), np.float32)
+        _chk(lib().cq_batch_read_p(self.h, C.c_int(i), out.ctypes.data_as(_F)))
+        return out
+
+    def enable_timing(self, on=True):
+        _chk(lib().cq_batch_enable_timing(self.h, C.c_int(1 if on else 0)))
+
+    def pass_times(self):
+        r, c, n = C.c_double(), C.c_double(), C.c_int()
+        _chk(lib().cq_batch_pass_times(self.h, C.byref(r), C.byref(c), C.byref(n)))
+        return r.value, c.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().cq_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))

@@ -1,0 +1,307 @@
+// cq_fft.h -- LDS-resident mixed-radix Stockham FFT for CDNA4 (gfx950).
+//
+// A workgroup transforms L lines of length n held in LDS (line stride LS
+// float2).  Each stage of radix R reads R strided inputs per butterfly into
+// registers, applies the stage twiddles from an LDS table of n roots
+// (w_n^j = exp(-2 pi i j / n), generated in double on the host), runs an
+// in-register DFT_R with compile-time constants, then writes the autosorted
+// outputs back to the same buffer after a barrier.  Supported radices:
+// 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17 (any length whose factors are these).
+// A complex multiplier can be fused into the last forward stage's store.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#ifndef CQ_NT
+#define CQ_NT 256
+#endif
+#define CQ_MAXP 32       // complex values a thread holds per stage
+#define CQ_MAXST 12      // max stages per plan
+
+struct CqPlan {
+    int n;
+    int ns;
+    int rad[CQ_MAXST];
+};
+
+// ---------------------------------------------------------------------------
+// compile-time trigonometry for the butterfly constants
+// ---------------------------------------------------------------------------
+constexpr double cq_pi = 3.14159265358979323846264338327950288;
+
+constexpr double cq_csin(double x)
+{
+    double x2 = x * x, term = x, sum = x;
+    for (int i = 1; i < 24; i++) {
+        term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cq_ccos(double x)
+{
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int i = 1; i < 24; i++) {
+        term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+// angle 2 pi k / R reduced to [-pi, pi]
+constexpr double cq_ang(int k, int R)
+{
+    int r = k % R;
+    if (2 * r > R) r -= R;
+    return 2.0 * cq_pi * (double)r / (double)R;
+}
+
+template <int R>
+struct CqRoots {
+    float c[R];
+    float s[R];
+    constexpr CqRoots() : c(), s()
+    {
+        for (int k = 0; k < R; k++) {
+            c[k] = (float)cq_ccos(cq_ang(k, R));
+            s[k] = (float)cq_csin(cq_ang(k, R));
+        }
+    }
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)  // a * conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 cmul_mi(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// in-register DFTs
+// ---------------------------------------------------------------------------
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = cmul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+// power-of-two radices from two half-size DFTs
+template <int R, bool INV>
+struct DftPow2 {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        constexpr int H = R / 2;
+        constexpr CqRoots<R> T{};
+        float2 e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        Dft<H, INV>::run(e);
+        Dft<H, INV>::run(o);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            float2 t;
+            if (k == 0) {
+                t = o[k];
+            } else if (4 * k == R) {
+                t = cmul_mi<INV>(o[k]);
+            } else {
+                float2 w = make_float2(T.c[k], INV ? T.s[k] : -T.s[k]);
+                t = cmul(o[k], w);
+            }
+            v[k] = cadd(e[k], t);
+            v[k + H] = csub(e[k], t);
+        }
+    }
+};
+template <bool INV>
+struct Dft<8, INV> : DftPow2<8, INV> {};
+template <bool INV>
+struct Dft<16, INV> : DftPow2<16, INV> {};
+
+// odd radices: pairwise symmetric form
+//   y_q = x0 + sum_j (x_j + x_{R-j}) cos(2 pi j q/R) -/+ i sum_j (x_j - x_{R-j}) sin(2 pi j q/R)
+template <int R, bool INV>
+struct DftOdd {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        constexpr int H = (R - 1) / 2;
+        constexpr CqRoots<R> T{};
+        float2 s[H], d[H];
+        float2 y0 = v[0];
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            s[j - 1] = cadd(v[j], v[R - j]);
+            d[j - 1] = csub(v[j], v[R - j]);
+            y0 = cadd(y0, s[j - 1]);
+        }
+        float2 out[R];
+        out[0] = y0;
+#pragma unroll
+        for (int q = 1; q <= H; q++) {
+            float2 A = v[0];
+            float2 B = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int j = 1; j <= H; j++) {
+                const int idx = (j * q) % R;
+                A.x += s[j - 1].x * T.c[idx];
+                A.y += s[j - 1].y * T.c[idx];
+                B.x += d[j - 1].x * T.s[idx];
+                B.y += d[j - 1].y * T.s[idx];
+            }
+            if (!INV) {
+                out[q] = make_float2(A.x + B.y, A.y - B.x);
+                out[R - q] = make_float2(A.x - B.y, A.y + B.x);
+            } else {
+                out[q] = make_float2(A.x - B.y, A.y + B.x);
+                out[R - q] = make_float2(A.x + B.y, A.y - B.x);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) v[q] = out[q];
+    }
+};
+template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
+template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
+template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
+template <bool INV> struct Dft<9, INV> : DftOdd<9, INV> {};
+template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
+template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
+template <bool INV> struct Dft<17, INV> : DftOdd<17, INV> {};
+
+// ---------------------------------------------------------------------------
+// one Stockham stage over L lines in LDS
+// ---------------------------------------------------------------------------
+// buf  : L lines, stride LS (float2)
+// tw   : LDS table, tw[j] = w_n^j
+// Ns   : product of the radices of the previous stages
+// mh   : optional multiplier (global, length n) applied to the outputs
+template <int R, bool INV>
+__device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
+                                         const float2 *tw, const float2 *__restrict__ mh)
+{
+    constexpr int NB = (CQ_MAXP / R) > 0 ? (CQ_MAXP / R) : 1;
+    const int m = n / R;
+    const int total = L * m;
+    const int twmul = n / (Ns * R);
+    float2 v[NB][R];
+    int dst[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int idx = threadIdx.x + b * CQ_NT;
+        dst[b] = -1;
+        if (idx < total) {
+            const int l = idx / m;
+            const int j = idx - l * m;
+            const int k = j % Ns;
+            const float2 *src = buf + l * LS + j;
+            const int tstep = k * twmul;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 x = src[r * m];
+                if (r > 0 && tstep != 0) {
+                    const float2 w = tw[r * tstep];
+                    x = INV ? cmulc(x, w) : cmul(x, w);
+                }
+                v[b][r] = x;
+            }
+            Dft<R, INV>::run(v[b]);
+            dst[b] = l * LS + (j - k) * R + k;
+            if (mh != nullptr) {
+                const int kk = (j - k) * R + k;
+#pragma unroll
+                for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk + q * Ns]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        if (dst[b] >= 0) {
+#pragma unroll
+            for (int q = 0; q < R; q++) buf[dst[b] + q * Ns] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool INV>
+__device__ __forceinline__ void cq_stage_rt(int R, float2 *buf, int LS, int n, int L, int Ns,
+                                            const float2 *tw, const float2 *__restrict__ mh)
+{
+    switch (R) {
+    case 2: cq_stage<2, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 3: cq_stage<3, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 4: cq_stage<4, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 5: cq_stage<5, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 7: cq_stage<7, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 8: cq_stage<8, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 9: cq_stage<9, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 11: cq_stage<11, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 13: cq_stage<13, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 16: cq_stage<16, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 17: cq_stage<17, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    default: break;
+    }
+}
+
+// Forward transform of L lines, optional multiplier fused into the last stage.
+__device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,
+                                           const float2 *tw, const float2 *__restrict__ mh)
+{
+    int Ns = 1;
+    for (int s = 0; s < p.ns; s++) {
+        const int R = p.rad[s];
+        cq_stage_rt<false>(R, buf, LS, p.n, L, Ns, tw, (s == p.ns - 1) ? mh : nullptr);
+        Ns *= R;
+    }
+}
+
+__device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS, int L,
+                                           const float2 *tw)
+{
+    int Ns = 1;
+    for (int s = 0; s < p.ns; s++) {
+        const int R = p.rad[s];
+        cq_stage_rt<true>(R, buf, LS, p.n, L, Ns, tw, nullptr);
+        Ns *= R;
+    }
+}
+
+// Largest number of complex points one workgroup can transform per stage for
+// a given radix (host and device agree on this).
+__host__ __device__ constexpr int cq_stage_capacity(int R)
+{
+    return CQ_NT * ((CQ_MAXP / R) > 0 ? (CQ_MAXP / R) : 1) * R;
+}

@@ -1,0 +1,110 @@
+/*
+ * cq.h -- C ABI of the MI355X constant-Q PSTD engine (libcq_hip.so).
+ *
+ * This is the drop-in boundary for the reference's per-binary GPU entry point
+ *     extern "C" void acforward(...)
+ * of sgpstd-qmodeling/acps4.cu:184-198 (aps), sgpstd-qmodeling/acsglw.cu:185-200
+ * (vaps), sgpstd-qrtm/acps4.cu:183-198 (artm) and sgpstd-qrtm/acsglw.cu:189-204
+ * (qrtm).  The reference passes one MultiGPU struct per device
+ * (sgpstd-qmodeling/statement.h:15-195) holding host arrays the driver filled
+ * and device pointers acforward allocates; here the driver describes each shot
+ * with a cq_job_desc of plain host pointers and sizes, and the engine owns all
+ * device memory.  No torch types, no exit(): every call returns a cq_status and
+ * cq_last_error() describes the last failure of the calling thread.
+ *
+ * Layout of every 2-D array: x-major with z fastest, a[ix * nzz + iz], the
+ * padded window of the reference (nxx = nx + 2 pml, nzz = nz + 2 pml).
+ * Gathers are time-major, rec[t * nr + i] (acps4.cu:620).
+ */
+#ifndef CQ_H
+#define CQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CQ_OK = 0,
+    CQ_EINVAL = -1,        /* bad argument */
+    CQ_EDEVICE = -2,       /* HIP runtime error */
+    CQ_ENOMEM = -3,        /* device allocation failed */
+    CQ_EUNSUPPORTED = -4,  /* e.g. an FFT length with an unsupported prime factor */
+    CQ_ESTATE = -5,        /* call out of order (e.g. reading results before running) */
+    CQ_ENUMERIC = -6       /* non-finite wavefield detected */
+} cq_status;
+
+typedef enum {
+    CQ_KIND_ACOUSTIC = 0,  /* aps : acps4.cu (modeling)               */
+    CQ_KIND_VISCO = 1,     /* vaps: acsglw.cu (constant-Q modeling)   */
+    CQ_KIND_RTM = 2,       /* artm: sgpstd-qrtm/acps4.cu              */
+    CQ_KIND_QRTM = 3       /* qrtm: sgpstd-qrtm/acsglw.cu             */
+} cq_kind;
+
+typedef struct cq_ctx_s *cq_ctx;
+typedef struct cq_batch_s *cq_batch;
+
+/* One shot.  All pointers are host memory owned by the caller and only read
+ * during cq_batch_create (outputs are written by cq_batch_read_*). */
+typedef struct {
+    int kind;                 /* cq_kind */
+    int nxx, nzz, pml;        /* padded window and CPML width */
+    int nt;                   /* time steps (tmax) */
+    float dt;                 /* seconds (the reference divides the ms value by 1000) */
+    /* window fields, nxx*nzz (select_part_model + expand_pml_areas output) */
+    const float *vp, *rho, *bbx, *bbz;
+    const float *gam, *lam, *vwd, *tao;   /* kind VISCO / QRTM (transfermodel) */
+    /* CPML profiles (get_pml_parameters): x arrays nxx, z arrays nzz */
+    const float *iax, *ibx, *hax, *hbx;
+    const float *iaz, *ibz, *haz, *hbz;
+    /* staggered shift multipliers (cal_shift_parameters) */
+    const float *kxsmr, *kxsmi, *kxscr, *kxsci;   /* nxx */
+    const float *kzsmr, *kzsmi, *kzscr, *kzsci;   /* nzz */
+    float dx, dz;             /* grid spacing (for the |k| tables) */
+    const float *rik;         /* source wavelet, nt (rik_cal) */
+    int xsg, zsg;             /* source grid index */
+    int nr;                   /* receivers */
+    const int *gxg, *gzg;     /* receiver grid indices, nr each */
+} cq_job_desc;
+
+/* Opens device `device` (HIP ordinal) and creates the engine's stream. */
+int cq_open(int device, cq_ctx *out);
+int cq_close(cq_ctx ctx);
+/* Text of the last error on this thread ("" if none). */
+const char *cq_last_error(void);
+/* Library version string. */
+const char *cq_version(void);
+
+/* Uploads njobs shots (models, coefficients, receiver maps) and zeroes their
+ * state.  Shots of different sizes and kinds may be mixed. */
+int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out);
+/* Enqueues nsteps time steps of every unfinished shot (asynchronous; shots
+ * whose nt is reached are skipped). */
+int cq_batch_step(cq_batch b, int nsteps);
+/* Waits for the batch's queued work. */
+int cq_batch_sync(cq_batch b);
+/* Runs every shot to its nt and waits. */
+int cq_batch_run(cq_batch b);
+/* Copies shot `job`'s gather (nt*nr floats) to host memory. */
+int cq_batch_read_rec(cq_batch b, int job, float *rec);
+/* Copies shot `job`'s pressure field (nxx*nzz floats) to host memory. */
+int cq_batch_read_p(cq_batch b, int job, float *p);
+/* Steps taken so far by the batch. */
+int cq_batch_time(cq_batch b);
+/* Device bytes held by the batch. */
+long long cq_batch_bytes(cq_batch b);
+int cq_batch_destroy(cq_batch b);
+
+/* The plain path: one shot, one launch sequence per time step, the gather
+ * row copied to the host and the stream synchronised after every step (the
+ * reference's per-step structure, acps4.cu:508-621).  rec receives nt*nr. */
+int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec);
+
+/* Average device time (ms) of the last `n` launches of the named pass
+ * ("row" or "col") recorded with HIP events when timing is enabled. */
+int cq_batch_enable_timing(cq_batch b, int on);
+int cq_batch_pass_times(cq_batch b, double *row_ms, double *col_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

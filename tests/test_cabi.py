@@ -1,0 +1,39 @@
+"""The C-ABI libraries load without a GPU and export every function that
+include/*.h declares (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "constant-q-pstd_amd")
+
+HEADERS = {"cq.h": "libcq_hip.so", "cq_host.h": "libcq_host.so", "cq_revolve.h": "libcq_host.so"}
+
+
+def declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(cq\w*)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_exports(header):
+    path = os.path.join(PKG, HEADERS[header])
+    if not os.path.exists(path):
+        pytest.fail("%s not built" % path)
+    lib = C.CDLL(path)
+    names = declared(header)
+    assert names, header
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_without_gpu():
+    lib = C.CDLL(os.path.join(PKG, "libcq_hip.so"))
+    lib.cq_version.restype = C.c_char_p
+    lib.cq_last_error.restype = C.c_char_p
+    assert lib.cq_version().startswith(b"cq-pstd")
+    assert lib.cq_last_error() == b""

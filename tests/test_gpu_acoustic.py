@@ -1,0 +1,117 @@
+"""Acoustic modeling on the MI355X through the C ABI, against the FP64 oracle
+(the reference's acps4.cu per-step kernels restated on the CPU).
+
+Tolerance: rel-L2 of the gather <= 1e-4 vs FP64 (the modeling bar);
+receiver and source indexing are integer and checked bit-exactly."""
+import numpy as np
+import pytest
+
+import orc
+from cqpy import engine, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+SHAPES = [
+    # nxt, nzt, pml, nt  -> padded (nxx, nzz) parities
+    (40, 24, 5, 200),     # 51 x 34  (3*17 odd, 2*17 even)
+    (41, 25, 7, 200),     # 55 x 39  (5*11, 3*13: odd, odd)
+    (52, 20, 6, 150),     # 65 x 32  (5*13, 2^5)
+    (60, 35, 8, 150),     # 77 x 51  (7*11, 3*17)
+    (56, 40, 12, 150),    # 81 x 64  (9*9, 16*4)
+]
+
+
+@pytest.mark.parametrize("nxt,nzt,pml,nt", SHAPES)
+def test_small_vs_oracle(ctx, nxt, nzt, pml, nt):
+    job = synth.small_job(nxt=nxt, nzt=nzt, pml=pml, nt=nt)
+    print("case", nxt, nzt, pml, nt, job["nxx"], job["nzz"], flush=True)
+    ref, st = orc.forward(job, want_state=True)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    e = engine.rel_l2(got, ref)
+    print("rel-L2 gather", e)
+    assert e <= TOL
+    ep = engine.rel_l2(b.p(0), st["p"])
+    print("rel-L2 final p", ep)
+    assert ep <= TOL
+    # the gather is p at the receiver cells: last row equals the final field there
+    p = b.p(0).reshape(job["nxx"], job["nzz"])
+    assert np.array_equal(got[-1], p[job["gxg"], job["gzg"]])
+    b.close()
+
+
+def test_plain_equals_batch(ctx):
+    job = synth.small_job(nxt=40, nzt=24, pml=5, nt=80)
+    b = ctx.batch([job])
+    b.run()
+    a = b.rec(0)
+    b.close()
+    p = ctx.plain_forward(job)
+    assert np.array_equal(a, p)
+
+
+def test_alone_equals_mixed_batch(ctx):
+    jobs = [synth.small_job(nxt=40, nzt=24, pml=5, nt=90, seed=1),
+            synth.small_job(nxt=61, nzt=32, pml=8, nt=60, seed=2, src_col=10),
+            synth.small_job(nxt=45, nzt=40, pml=5, nt=120, seed=3, nrec=17)]
+    alone = []
+    for j in jobs:
+        b = ctx.batch([j])
+        b.run()
+        alone.append(b.rec(0))
+        b.close()
+    b = ctx.batch(jobs)
+    b.run()
+    for i, j in enumerate(jobs):
+        assert np.array_equal(b.rec(i), alone[i]), i
+    b.close()
+
+
+def test_duplicate_and_clamped_receivers(ctx):
+    job = synth.small_job(nxt=40, nzt=24, pml=5, nt=50)
+    job["gxg"] = np.array([5, 5, 20, 45, 45, 10], np.int32)   # duplicates and PML-edge cells
+    job["gzg"] = np.array([7, 7, 7, 7, 8, 28], np.int32)
+    ref = orc.forward(job)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    assert engine.rel_l2(got, ref) <= TOL
+    assert np.array_equal(got[:, 0], got[:, 1])
+    b.close()
+
+
+def test_config1_full_vs_oracle(ctx):
+    """Plain acoustic workload at full size: 459 x 220 padded, nt = 3000."""
+    job = synth.config1_job(nt=3000)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    orc.set_threads(16)
+    ref = orc.forward(job)
+    e = engine.rel_l2(got, ref)
+    print("config1 rel-L2 vs FP64 oracle:", e)
+    assert e <= TOL
+
+
+def test_bad_inputs_raise(ctx):
+    job = synth.small_job(nxt=40, nzt=24, pml=5, nt=10)
+    bad = dict(job)
+    bad["xsg"] = job["nxx"] + 3
+    with pytest.raises(engine.CqError):
+        ctx.batch([bad])
+    bad = dict(job)
+    bad["nxx"] = 61 * 2  # not matching arrays but also a prime factor 61
+    with pytest.raises(engine.CqError):
+        ctx.batch([bad])
