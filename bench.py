@@ -30,17 +30,40 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(jobs):
-    """Minimal-traffic bytes per time step for the two acoustic passes.
-    Row pass: reads div, Lz, p, c1, c2, vx, dt*bx; writes p, vx, dvx
-    (40 B/cell) + x-CPML strips (phi, psi read+write: 16 B per strip cell).
-    Column pass: reads p, vz, dt*bz, dvx; writes vz, div, Lz (28 B/cell) +
-    z-CPML strips (16 B per strip cell)."""
-    row = col = 0
+    """Minimal-traffic bytes per launch of each pass, summed over the shots it
+    serves.  A cell is a padded-grid point (nxx*nzz); a half-spectrum word is
+    one complex kz sample per column (nxx*(nzz/2+1) complex = 8 B each).
+    Acoustic row pass: reads div, Lz, p, c1, c2, vx, dt*bx; writes p, vx,
+      dvx (40 B/cell) + x-CPML strips (phi, psi read+write, 16 B per strip cell).
+    Acoustic column pass: reads p, vz, dt*bz, dvx; writes vz, div, Lz
+      (28 B/cell) + z-CPML strips (16 B per strip cell).
+    Viscoacoustic: C1 reads p, vz, dt*bz, dvx, writes vz, div (24 B/cell) and
+      2 half spectra; R1 reads 2, writes 4 half spectra; C2 reads 3 half
+      spectra, div, gam, y2, c (16 B/cell), writes total (4 B/cell) and 1
+      half spectrum; R2 reads 1, writes 1; C3 reads 2 half spectra, p, total,
+      3 coefficients (24 B/cell), writes p (4 B/cell); R3 reads p, vx, dt*bx,
+      writes vx, dvx (20 B/cell) + strips."""
+    out = {}
+
+    def add(k, v):
+        out[k] = out.get(k, 0) + v
+
     for j in jobs:
         nxx, nzz, pml = j["nxx"], j["nzz"], j["pml"]
-        row += 40 * nxx * nzz + 16 * 2 * pml * nzz
-        col += 28 * nxx * nzz + 16 * 2 * pml * nxx
-    return row, col
+        n = nxx * nzz
+        hs = nxx * (nzz // 2 + 1) * 8
+        sx, sz = 16 * 2 * pml * nzz, 16 * 2 * pml * nxx
+        if j["kind"] == 0:
+            add("k_row_ac", 40 * n + sx)
+            add("k_col_ac", 28 * n + sz)
+        else:
+            add("k_q_c1", 24 * n + 2 * hs + sz)
+            add("k_q_r1", 6 * hs)
+            add("k_q_c2", 20 * n + 4 * hs)
+            add("k_q_r2", 2 * hs)
+            add("k_q_c3", 28 * n + 2 * hs)
+            add("k_q_r3", 20 * n + sx)
+    return out
 
 
 def cpu_baseline(job, steps, threads):
@@ -67,10 +90,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batch128", choices=["batch128", "config1"])
     ap.add_argument("--shots", type=int, default=128)
+    ap.add_argument("--kinds", default="mixed", choices=["mixed", "acoustic", "visco"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a streaming copy of this many GB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one "
@@ -79,13 +105,26 @@ def main():
 
     from cqpy import engine, synth
 
+    if args.copy_calibration > 0:
+        with engine.Context(0) as c:
+            nb = int(args.copy_calibration * 1e9)
+            ms = c.copy_calibrate(nb, reps=5)
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "ms_per_copy": ms,
+                          "GBs_read_plus_write": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
+
     nt_total = args.warmup + args.steps
     if args.workload == "batch128":
-        jobs = synth.batch128_jobs(nt=max(nt_total, 5000), nshots=args.shots)
+        kinds = [i % 2 if args.kinds == "mixed" else (0 if args.kinds == "acoustic" else 1)
+                 for i in range(args.shots)]
+        jobs = synth.batch128_jobs(nt=max(nt_total, 5000), nshots=args.shots, kinds=kinds)
         wl = {"workload": "batch128: %d shots, 600x2400 model, 40-pt CPML, dx=dz=5 m, "
                           "dt=0.4 ms, windows cropped to +-600 cells (uneven, widened to "
                           "FFT-friendly sizes)" % args.shots,
-              "shots": args.shots, "kinds": "acoustic",
+              "shots": args.shots,
+              "kinds": {"mixed": "half acoustic, half viscoacoustic (rank-4 fractional "
+                                 "Laplacian)", "acoustic": "acoustic",
+                        "visco": "viscoacoustic"}[args.kinds],
               "cells_per_step": int(sum(j["nxx"] * j["nzz"] for j in jobs))}
     else:
         jobs = [synth.config1_job(nt=max(nt_total, 3000), want_k2=False)]
@@ -107,16 +146,20 @@ def main():
 
     # per-pass device time with HIP events on the engine's stream
     b.enable_timing(True)
-    nprof = min(20, jobs[0]["nt"] - b.time())
-    b.step(nprof)
-    row_ms, col_ms, nev = b.pass_times()
+    b.step(min(10, jobs[0]["nt"] - b.time()))
+    kt = b.kernel_times()
     b.enable_timing(False)
-    rowB, colB = algorithmic_bytes(jobs)
-    row_gbs = rowB / (row_ms * 1e-3) / 1e9 if row_ms > 0 else 0.0
-    col_gbs = colB / (col_ms * 1e-3) / 1e9 if col_ms > 0 else 0.0
-    dom = "row" if row_ms >= col_ms else "col"
-    achieved = row_gbs if dom == "row" else col_gbs
     b.close()
+    algo = algorithmic_bytes(jobs)
+    passes = {}
+    for name, (ms, calls) in kt.items():
+        gbs = algo[name] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        passes[name] = {"ms": round(ms, 4), "bytes": algo[name], "GBs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(kt, key=lambda k: kt[k][0])
+    achieved = passes[dom]["GBs"]
+    step_bytes = sum(algo.values())
+    print("passes:", json.dumps(passes), file=sys.stderr, flush=True)
 
     # plain path: one shot of the workload, per-step gather copy + sync
     plain = None
@@ -134,11 +177,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        mid = jobs[len(jobs) // 2]
+        mid = dict(jobs[len(jobs) // 2])
+        mid["kind"] = 0
         v, sec = cpu_baseline(mid, args.cpu_steps, threads)
         cpu = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-               "sample": "FP32 reference-faithful CPU path (9 full 2-D C2C FFTs per step, "
-                         "OpenMP) on one %dx%d shot of the workload, %d steps, %.1f s"
+               "sample": "FP32 reference-faithful CPU path (acoustic: 9 full 2-D C2C FFTs per "
+                         "step, OpenMP) on one %dx%d shot of the workload, %d steps, %.1f s"
                          % (mid["nxx"], mid["nzz"], args.cpu_steps, sec)}
 
     out = {
@@ -157,12 +201,11 @@ def main():
         "config": wl,
         "plain_mcell_steps_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_row_ac" if dom == "row" else "k_col_ac",
-                     "row_ms": round(row_ms, 4), "col_ms": round(col_ms, 4),
-                     "row_bytes": rowB, "col_bytes": colB,
-                     "row_GBs": round(row_gbs, 1), "col_GBs": round(col_gbs, 1)},
+                     "kernel": dom, "passes": passes,
+                     "step_bytes": step_bytes,
+                     "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

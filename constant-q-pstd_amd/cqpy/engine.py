@@ -33,7 +33,8 @@ class JobDesc(C.Structure):
 SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_create",
            "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",
            "cq_batch_read_p", "cq_batch_time", "cq_batch_bytes", "cq_batch_destroy",
-           "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times"]
+           "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times",
+           "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name"]
 
 _lib = None
 
@@ -52,6 +53,7 @@ def lib():
         L.cq_last_error.restype = C.c_char_p
         L.cq_version.restype = C.c_char_p
         L.cq_batch_bytes.restype = C.c_longlong
+        L.cq_pass_name.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -116,6 +118,12 @@ class Context:
     def batch(self, jobs):
         return Batch(self, jobs)
 
+    def copy_calibrate(self, nbytes, reps=10):
+        """ms per streaming copy of nbytes (reads nbytes, writes nbytes)."""
+        ms = C.c_double()
+        _chk(lib().cq_copy_calibrate(self.h, C.c_longlong(int(nbytes)), C.c_int(reps), C.byref(ms)))
+        return ms.value
+
 
 class Batch:
     def __init__(self, ctx, jobs):
@@ -163,6 +171,18 @@ class Batch:
         r, c, n = C.c_double(), C.c_double(), C.c_int()
         _chk(lib().cq_batch_pass_times(self.h, C.byref(r), C.byref(c), C.byref(n)))
         return r.value, c.value, n.value
+
+    def kernel_times(self):
+        """{kernel name: (average ms per launch, launches)} for the passes
+        timed while timing was enabled."""
+        ms = (C.c_double * 8)()
+        calls = (C.c_int * 8)()
+        _chk(lib().cq_batch_kernel_times(self.h, ms, calls))
+        out = {}
+        for p in range(8):
+            if calls[p]:
+                out[lib().cq_pass_name(C.c_int(p)).decode()] = (ms[p], calls[p])
+        return out
 
     def close(self):
         if self.h:

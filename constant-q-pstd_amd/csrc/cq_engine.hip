@@ -1,28 +1,26 @@
 // cq_engine.hip -- MI355X (gfx950) engine for staggered-grid pseudo-spectral
-// time stepping: acoustic modeling with CPML (the reference's acps4.cu path).
+// time stepping: acoustic (acps4.cu) and fractional-Laplacian constant-Q
+// (acsglw.cu) modeling with CPML, behind the C ABI of include/cq.h.
 //
-// Design (vs the reference's 9 full 2-D C2C FFTs and 10 elementwise kernels
-// per step, sgpstd-qmodeling/acps4.cu:514-607):
-//   * Every acoustic operator is separable: the staggered derivatives depend on
-//     kx or kz alone and k^2 = kx^2 + kz^2, so the 2-D transforms reduce
-//     exactly to 1-D transforms along x or z.  The reference keeps only the
-//     real part of each complex result; for a real input that equals a 1-D
-//     transform with the Hermitian-symmetrised multiplier
-//     m_h(k) = (m(k) + conj(m(-k)))/2, which also reproduces the reference's
-//     odd-n half-cell phase quirk (funca.c:394-395) exactly.
-//   * With Hermitian multipliers two real lines ride in one complex FFT
-//     (real and imaginary parts), halving the transform work.
-//   * One time step is two fused passes:
-//       row pass (x lines, z tiles):  L = Lx + Lz, p update + source,
-//                                     dp/dx, x-CPML, vx update, dvx/dx + x-CPML
-//       col pass (z lines, x tiles):  receiver gather, dp/dz, z-CPML, vz update,
-//                                     dvz/dz + z-CPML, div, Lz = d2/dz2 div
-//     reading 11 and writing 6 float words per cell (68 B/cell-step plus the
-//     CPML strips), all FFT work staying in LDS.
+// Design (vs the reference's 9 (acoustic) / 14 (viscoacoustic) full 2-D C2C
+// FFTs and 10-17 elementwise kernels per step, sgpstd-qmodeling/acps4.cu:514-607,
+// acsglw.cu:517-645):
+//   * The staggered derivatives depend on kx or kz alone and k^2 = kx^2 + kz^2,
+//     so those 2-D transforms reduce exactly to 1-D transforms along x or z.
+//     The reference keeps only the real part of each complex result; for a real
+//     input that equals a 1-D transform with the Hermitian-symmetrised
+//     multiplier m_h(k) = (m(k) + conj(m(-k)))/2, which also reproduces the
+//     reference's odd-n half-cell phase quirk (funca.c:394-395) exactly.
+//   * Hermitian multipliers let two real lines ride in one complex FFT.
+//   * Acoustic step = 2 fused passes (row: p update, vx; column: vz, div, Lz),
+//     68 B/cell-step of HBM traffic plus the CPML strips.
+//   * Viscoacoustic step = 6 fused passes over real fields and kz half spectra
+//     (see cq_passes.h), about 152 B/cell-step.
 //   * CPML memory variables are zero outside the absorbing strips (a = 0
 //     there), so they are stored for the strips only.
-//   * Many shots of different sizes advance in the same launches: the grid is a
-//     list of (shot, tile) work items over a device table of shots.
+//   * Shots of different sizes and kinds advance in the same launches: the
+//     grid is a list of (shot, tile) work items over a device table of shots,
+//     ordered so neighbouring tiles run on the same XCD.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +34,9 @@
 #include <vector>
 
 #include "cq.h"
-#include "cq_fft.h"
+#include "cq_passes.h"
 
-#define CQ_VERSION "cq-pstd-mi355x 0.1"
+#define CQ_VERSION "cq-pstd-mi355x 0.2"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -64,274 +62,11 @@ static int set_err(int code, const char *fmt, ...)
             return set_err(CQ_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-// ---------------------------------------------------------------------------
-// device-side shot table
-// ---------------------------------------------------------------------------
-struct JobD {
-    int kind, nxx, nzz, ldz, pml, nt, xsg, zsg, nr;
-    float src_scale;  // dt * 1000 (acps4.cu:1491)
-    CqPlan px, pz;
-    const float2 *twx, *twz;
-    const float2 *mxm, *mxp, *mx2, *mzm, *mzp, *mz2;
-    float *p, *vx, *vz, *div, *lz, *dvx;
-    const float *c1, *c2, *dtbx, *dtbz;
-    float *psix, *phix, *psiz, *phiz;
-    const float *iax, *ibx, *hax, *hbx, *iaz, *ibz, *haz, *hbz;
-    const float *rik;
-    float *rec;
-    const int *rcv_x, *rcv_z, *rcv_i, *rcv_off;
-};
-
-__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
-__device__ __forceinline__ float f4get(const float4 &v, int k)
+// Streaming copy used to calibrate HBM byte counters and bandwidth.
+__global__ __launch_bounds__(256) void k_copy_f4(const float4 *__restrict__ a, float4 *__restrict__ b, long long n4)
 {
-    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
-}
-__device__ __forceinline__ void f4set(float4 &v, int k, float s)
-{
-    if (k == 0) v.x = s;
-    else if (k == 1) v.y = s;
-    else if (k == 2) v.z = s;
-    else v.w = s;
-}
-
-// ---------------------------------------------------------------------------
-// acoustic row pass (x lines).  Work item = (shot, z0); the tile is TZ
-// consecutive depths (TZ/2 packed complex lines of length nxx).
-// ---------------------------------------------------------------------------
-template <int TZ>
-__global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
-                                                  const int2 *__restrict__ items, int step)
-{
-    constexpr int L = TZ / 2;
-    constexpr int Q = TZ / 4;
-    const int2 it = items[blockIdx.x];
-    const JobD &J = jobs[it.x];
-    const int t = step;
-    if (t >= J.nt) return;
-    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y, pml = J.pml;
-    extern __shared__ float2 lds[];
-    float2 *tw = lds;
-    float2 *buf = lds + nxx;
-    float4 *sav = reinterpret_cast<float4 *>(buf + L * nxx);
-    for (int i = threadIdx.x; i < nxx; i += CQ_NT) tw[i] = J.twx[i];
-    const int ne = nxx * Q;
-
-    // phase 1: div -> LDS (kept for the p update)
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const float4 d = ld4(J.div + (size_t)ix * ldz + z0 + 4 * q);
-        sav[e] = d;
-        buf[(2 * q) * nxx + ix] = make_float2(d.x, d.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(d.z, d.w);
-    }
-    __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mx2);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
-
-    // phase 2: p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498)
-    const float src = J.rik[t] * J.src_scale;
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
-        const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g),
-                     lz = ld4(J.lz + g), d = sav[e];
-        float4 pn;
-        pn.x = p.x + c1.x * d.x - c2.x * (a.x + lz.x);
-        pn.y = p.y + c1.y * d.y - c2.y * (a.y + lz.y);
-        pn.z = p.z + c1.z * d.z - c2.z * (b.x + lz.z);
-        pn.w = p.w + c1.w * d.w - c2.w * (b.y + lz.w);
-        if (ix == J.xsg) {
-            const int dz = J.zsg - (z0 + 4 * q);
-            if (dz >= 0 && dz < 4) f4set(pn, dz, f4get(pn, dz) + src);
-        }
-        st4(J.p + g, pn);
-        buf[(2 * q) * nxx + ix] = make_float2(pn.x, pn.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(pn.z, pn.w);
-    }
-    __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
-
-    // phase 3: x-CPML on dp/dx and vx update (vx_cal_kernel, acps4.cu:1274-1303)
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
-        float4 dp = make_float4(a.x, a.y, b.x, b.y);
-        const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
-        if (ix < pml || ix >= nxx - pml) {
-            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float *php = J.phix + (size_t)sx * ldz + z0 + 4 * q;
-            const float hb = J.hbx[ix], ha = J.hax[ix];
-            float4 ph = ld4(php);
-            ph.x = hb * ph.x + ha * dp.x;
-            ph.y = hb * ph.y + ha * dp.y;
-            ph.z = hb * ph.z + ha * dp.z;
-            ph.w = hb * ph.w + ha * dp.w;
-            st4(php, ph);
-            dp.x += ph.x; dp.y += ph.y; dp.z += ph.z; dp.w += ph.w;
-        }
-        float4 vn;
-        vn.x = vx.x + bx.x * dp.x;
-        vn.y = vx.y + bx.y * dp.y;
-        vn.z = vx.z + bx.z * dp.z;
-        vn.w = vx.w + bx.w * dp.w;
-        st4(J.vx + g, vn);
-        buf[(2 * q) * nxx + ix] = make_float2(vn.x, vn.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(vn.z, vn.w);
-    }
-    __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
-
-    // phase 4: dvx/dx for the next step with its x-CPML (update_vxx_vzz, :1111-1144)
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
-        float4 dv = make_float4(a.x, a.y, b.x, b.y);
-        if (ix < pml || ix >= nxx - pml) {
-            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float *psp = J.psix + (size_t)sx * ldz + z0 + 4 * q;
-            const float ib = J.ibx[ix], ia = J.iax[ix];
-            float4 ps = ld4(psp);
-            ps.x = ib * ps.x + ia * dv.x;
-            ps.y = ib * ps.y + ia * dv.y;
-            ps.z = ib * ps.z + ia * dv.z;
-            ps.w = ib * ps.w + ia * dv.w;
-            st4(psp, ps);
-            dv.x += ps.x; dv.y += ps.y; dv.z += ps.z; dv.w += ps.w;
-        }
-        st4(J.dvx + g, dv);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// acoustic column pass (z lines).  Work item = (shot, x0); CW columns packed
-// in CW/2 complex lines of length nzz.
-// ---------------------------------------------------------------------------
-template <int CW>
-__global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
-                                                  const int2 *__restrict__ items, int step)
-{
-    constexpr int L = CW / 2;
-    const int2 it = items[blockIdx.x];
-    const JobD &J = jobs[it.x];
-    const int t = step;
-    if (t >= J.nt) return;
-    const int nxx = J.nxx, nzz = J.nzz, ldz = J.ldz, x0 = it.y, pml = J.pml;
-    const int nq = (nzz + 3) >> 2;
-    extern __shared__ float2 lds[];
-    float2 *tw = lds;
-    float2 *buf = lds + nzz;
-    float *bufs = reinterpret_cast<float *>(buf);
-    for (int i = threadIdx.x; i < nzz; i += CQ_NT) tw[i] = J.twz[i];
-    const int ne = CW * nq;
-
-    // phase 1: p -> LDS
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ix < nxx) v = ld4(J.p + (size_t)ix * ldz + 4 * zq);
-        float *line = bufs + 2 * (c >> 1) * nzz + (c & 1);
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
-    }
-    __syncthreads();
-    // receiver gather of the updated p (acps4.cu:611-621)
-    {
-        const int tile = x0 / CW;
-        const int r0 = J.rcv_off[tile], r1 = J.rcv_off[tile + 1];
-        for (int r = r0 + threadIdx.x; r < r1; r += CQ_NT) {
-            const int c = J.rcv_x[r] - x0, z = J.rcv_z[r];
-            J.rec[(size_t)t * J.nr + J.rcv_i[r]] = bufs[2 * ((c >> 1) * nzz + z) + (c & 1)];
-        }
-    }
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzp);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
-
-    // phase 2: z-CPML on dp/dz and vz update (vz_cal_kernel, acps4.cu:1305-1333)
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
-        if (ix >= nxx) continue;
-        const size_t g = (size_t)ix * ldz + 4 * zq;
-        float *line = bufs + 2 * (c >> 1) * nzz + (c & 1);
-        const float4 vz = ld4(J.vz + g), bz = ld4(J.dtbz + g);
-        float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            float dp = line[2 * iz];
-            if (iz < pml || iz >= nzz - pml) {
-                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float *php = J.phiz + (size_t)ix * (2 * pml) + sz;
-                const float ph = J.hbz[iz] * (*php) + J.haz[iz] * dp;
-                *php = ph;
-                dp += ph;
-            }
-            const float v = f4get(vz, k) + f4get(bz, k) * dp;
-            f4set(vn, k, v);
-            line[2 * iz] = v;
-        }
-        st4(J.vz + g, vn);
-    }
-    __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
-
-    // phase 3: dvz/dz with z-CPML; div = dvx + dvz (update_vxx_vzz + sumpxpz)
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
-        float *line = bufs + 2 * (c >> 1) * nzz + (c & 1);
-        if (ix >= nxx) {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
-            continue;
-        }
-        const size_t g = (size_t)ix * ldz + 4 * zq;
-        const float4 dvx = ld4(J.dvx + g);
-        float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            float d = line[2 * iz];
-            if (iz < pml || iz >= nzz - pml) {
-                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
-                const float ps = J.ibz[iz] * (*psp) + J.iaz[iz] * d;
-                *psp = ps;
-                d += ps;
-            }
-            const float s = f4get(dvx, k) + d;
-            f4set(dv, k, s);
-            line[2 * iz] = s;
-        }
-        st4(J.div + g, dv);
-    }
-    __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mz2);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
-
-    // phase 4: Lz = d2/dz2 div
-    for (int e = threadIdx.x; e < ne; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
-        if (ix >= nxx) continue;
-        const float *line = bufs + 2 * (c >> 1) * nzz + (c & 1);
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(o, k, line[2 * iz]);
-        }
-        st4(J.lz + (size_t)ix * ldz + 4 * zq, o);
-    }
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -348,7 +83,6 @@ struct cq_ctx_s {
     }
 };
 
-
 // Factorisation into supported radices: powers of two as 16/8/4/2, powers of
 // three as 9/3, then 5, 7, 11, 13, 17.
 static bool make_plan(int n, CqPlan &p)
@@ -356,12 +90,12 @@ static bool make_plan(int n, CqPlan &p)
     memset(&p, 0, sizeof p);
     p.n = n;
     int m = n;
-    while (m % 16 == 0) { p.rad[p.ns++] = 16; m /= 16; }
+    while (m % 16 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 16; m /= 16; }
     if (m % 8 == 0) { p.rad[p.ns++] = 8; m /= 8; }
     if (m % 4 == 0) { p.rad[p.ns++] = 4; m /= 4; }
     if (m % 2 == 0) { p.rad[p.ns++] = 2; m /= 2; }
-    while (m % 9 == 0) { p.rad[p.ns++] = 9; m /= 9; }
-    while (m % 3 == 0) { p.rad[p.ns++] = 3; m /= 3; }
+    while (m % 9 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 9; m /= 9; }
+    while (m % 3 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 3; m /= 3; }
     for (int r : {5, 7, 11, 13, 17})
         while (m % r == 0) {
             if (p.ns >= CQ_MAXST) return false;
@@ -404,9 +138,8 @@ static int get_plan(cq_ctx ctx, int n, CqPlan &p, const float2 *&tw)
 
 // Symmetrised staggered-derivative multiplier / n for one axis:
 // m(k) = +-i (kr + i ki), sign + for k <= n/2 (shift_vxz_kernel, acps4.cu:1069-1105)
-static void deriv_table(int n, const float *kr, const float *ki, std::vector<float2> &out)
+static void deriv_table(int n, const float *kr, const float *ki, float2 *out)
 {
-    out.resize(n);
     auto m = [&](int k) {
         const double s = (k <= n / 2) ? 1.0 : -1.0;
         // s * i * (kr + i ki) = s * (-ki + i kr)
@@ -421,11 +154,13 @@ static void deriv_table(int n, const float *kr, const float *ki, std::vector<flo
     }
 }
 
+// Wavenumber step of the reference's float grid (funca.c:363)
+static float k_step(int n, float d) { return (float)(2 * 3.1415926 / (n * d)); }
+
 // |k|^2 / n with the reference's float wavenumber grid (funca.c:552-567)
-static void lap_table(int n, float d, std::vector<float2> &out)
+static void lap_table(int n, float d, float2 *out)
 {
-    out.resize(n);
-    const float dk = 2 * 3.1415926 / (n * d);
+    const float dk = k_step(n, d);
     std::vector<float> kk(n);
     const int nh = n / 2 + 1;
     for (int i = 0; i < nh; i++) kk[i] = i * dk;
@@ -448,26 +183,32 @@ struct JobH {
     JobD d;
     void *mem = nullptr;
     size_t bytes = 0;
-    int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0;
+    int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0, kind = 0;
 };
+
+// One launch of the per-step sequence.
+enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_QC2, P_QR2, P_QC3, P_QR3, P_COUNT };
+static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1",
+                                         "k_q_c2",   "k_q_r2",   "k_q_c3", "k_q_r3"};
 
 struct cq_batch_s {
     cq_ctx ctx = nullptr;
     std::vector<JobH> jobs;
     JobD *djobs = nullptr;
-    int2 *ditems_row = nullptr, *ditems_col = nullptr;
-    int nrow = 0, ncol = 0;
-    int tz = 16, cw = 16;
-    size_t lds_row = 0, lds_col = 0;
+    int2 *ditems[P_COUNT] = {};
+    int nitems[P_COUNT] = {};
+    size_t lds[P_COUNT] = {};
+    int tz = 16, cw = 16, tk = 8;
     int t = 0, tmax = 0;
     bool timing = false;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;        // per step: start + one per launch
+    std::vector<int> ev_pass;          // pass id of each event after the start
+    std::vector<int> step_nev;         // launches recorded per step
     ~cq_batch_s()
     {
         for (auto &j : jobs) (void)hipFree(j.mem);
         (void)hipFree(djobs);
-        (void)hipFree(ditems_row);
-        (void)hipFree(ditems_col);
+        for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[i]);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
@@ -505,8 +246,8 @@ extern "C" int cq_close(cq_ctx ctx)
 
 static int validate(const cq_job_desc &j, int idx)
 {
-    if (j.kind != CQ_KIND_ACOUSTIC)
-        return set_err(CQ_EUNSUPPORTED, "job %d: kind %d not supported by this build", idx, j.kind);
+    if (j.kind != CQ_KIND_ACOUSTIC && j.kind != CQ_KIND_VISCO)
+        return set_err(CQ_EUNSUPPORTED, "job %d: kind %d is not a modeling kind", idx, j.kind);
     if (j.nxx < 8 || j.nzz < 8 || j.pml < 2 || 2 * j.pml > j.nxx || 2 * j.pml > j.nzz)
         return set_err(CQ_EINVAL, "job %d: bad window %dx%d pml %d", idx, j.nxx, j.nzz, j.pml);
     if (j.nt < 1 || !(j.dt > 0.f)) return set_err(CQ_EINVAL, "job %d: bad nt/dt", idx);
@@ -514,6 +255,8 @@ static int validate(const cq_job_desc &j, int idx)
         !j.iaz || !j.ibz || !j.haz || !j.hbz || !j.kxsmr || !j.kxsmi || !j.kxscr || !j.kxsci ||
         !j.kzsmr || !j.kzsmi || !j.kzscr || !j.kzsci)
         return set_err(CQ_EINVAL, "job %d: missing input array", idx);
+    if (j.kind == CQ_KIND_VISCO && (!j.gam || !j.lam || !j.vwd || !j.tao))
+        return set_err(CQ_EINVAL, "job %d: viscoacoustic job without gam/lam/vwd/tao", idx);
     if (j.xsg < 0 || j.xsg >= j.nxx || j.zsg < 0 || j.zsg >= j.nzz)
         return set_err(CQ_EINVAL, "job %d: source index outside the window", idx);
     if (j.nr < 0 || (j.nr > 0 && (!j.gxg || !j.gzg)))
@@ -525,17 +268,24 @@ static int validate(const cq_job_desc &j, int idx)
     return CQ_OK;
 }
 
-static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
+static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
 {
     cq_ctx ctx = b->ctx;
+    const int cw = b->cw, tk = b->tk;
+    const bool q = j.kind == CQ_KIND_VISCO;
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = (nzz + 31) & ~31;
+    const int nzh = nzz / 2 + 1;
+    const int ldk = (nzh + tk - 1) / tk * tk;
     const size_t N = (size_t)nxx * ldz;
     JobD &d = h.d;
     memset(&d, 0, sizeof d);
     d.kind = j.kind; d.nxx = nxx; d.nzz = nzz; d.ldz = ldz; d.pml = pml; d.nt = j.nt;
-    d.xsg = j.xsg; d.zsg = j.zsg; d.nr = j.nr;
+    d.xsg = j.xsg; d.zsg = j.zsg; d.nr = j.nr; d.nzh = nzh; d.ldk = ldk;
     d.src_scale = j.dt * 1000.0f;
+    d.dkx = k_step(nxx, j.dx);
+    d.dkz = k_step(nzz, j.dz);
+    d.invN = (float)(1.0 / ((double)nxx * nzz));
     int rc;
     if ((rc = get_plan(ctx, nxx, d.px, d.twx)) != CQ_OK) return rc;
     if ((rc = get_plan(ctx, nzz, d.pz, d.twz)) != CQ_OK) return rc;
@@ -543,9 +293,15 @@ static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
     const int ntile_x = (nxx + cw - 1) / cw;
     Arena A;
     const size_t o_p = A.take<float>(N), o_vx = A.take<float>(N), o_vz = A.take<float>(N);
-    const size_t o_div = A.take<float>(N), o_lz = A.take<float>(N), o_dvx = A.take<float>(N);
-    const size_t o_c1 = A.take<float>(N), o_c2 = A.take<float>(N);
+    const size_t o_div = A.take<float>(N), o_dvx = A.take<float>(N);
     const size_t o_bx = A.take<float>(N), o_bz = A.take<float>(N);
+    // acoustic: lz, c1, c2 ; viscoacoustic: tot, qa, qb, qe, qg, qy2, qcc, 4 half spectra
+    const size_t o_f1 = A.take<float>(N), o_f2 = A.take<float>(N), o_f3 = A.take<float>(N);
+    size_t o_q[4] = {0, 0, 0, 0}, o_s[4] = {0, 0, 0, 0};
+    if (q) {
+        for (int i = 0; i < 4; i++) o_q[i] = A.take<float>(N);
+        for (int i = 0; i < 4; i++) o_s[i] = A.take<float2>((size_t)nxx * ldk);
+    }
     const size_t o_psix = A.take<float>((size_t)2 * pml * ldz), o_phix = A.take<float>((size_t)2 * pml * ldz);
     const size_t o_psiz = A.take<float>((size_t)2 * pml * nxx), o_phiz = A.take<float>((size_t)2 * pml * nxx);
     const size_t o_1x = A.take<float>((size_t)4 * nxx), o_1z = A.take<float>((size_t)4 * nzz);
@@ -560,13 +316,22 @@ static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
     if (e != hipSuccess) return set_err(CQ_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e));
     h.mem = mem;
     h.bytes = total;
-    h.nt = j.nt; h.nr = j.nr; h.nxx = nxx; h.nzz = nzz; h.ldz = ldz;
+    h.nt = j.nt; h.nr = j.nr; h.nxx = nxx; h.nzz = nzz; h.ldz = ldz; h.kind = j.kind;
     HIPCHK(hipMemsetAsync(mem, 0, total, ctx->stream));
 
     d.p = (float *)(mem + o_p); d.vx = (float *)(mem + o_vx); d.vz = (float *)(mem + o_vz);
-    d.div = (float *)(mem + o_div); d.lz = (float *)(mem + o_lz); d.dvx = (float *)(mem + o_dvx);
-    d.c1 = (float *)(mem + o_c1); d.c2 = (float *)(mem + o_c2);
+    d.div = (float *)(mem + o_div); d.dvx = (float *)(mem + o_dvx);
     d.dtbx = (float *)(mem + o_bx); d.dtbz = (float *)(mem + o_bz);
+    float *f1 = (float *)(mem + o_f1), *f2 = (float *)(mem + o_f2), *f3 = (float *)(mem + o_f3);
+    if (!q) {
+        d.lz = f1; d.c1 = f2; d.c2 = f3;
+    } else {
+        d.tot = f1; d.qa = f2; d.qb = f3;
+        d.qe = (float *)(mem + o_q[0]); d.qg = (float *)(mem + o_q[1]);
+        d.qy2 = (float *)(mem + o_q[2]); d.qcc = (float *)(mem + o_q[3]);
+        d.s1 = (float2 *)(mem + o_s[0]); d.s2 = (float2 *)(mem + o_s[1]);
+        d.s3 = (float2 *)(mem + o_s[2]); d.s4 = (float2 *)(mem + o_s[3]);
+    }
     d.psix = (float *)(mem + o_psix); d.phix = (float *)(mem + o_phix);
     d.psiz = (float *)(mem + o_psiz); d.phiz = (float *)(mem + o_phiz);
     float *x1 = (float *)(mem + o_1x), *z1 = (float *)(mem + o_1z);
@@ -581,38 +346,48 @@ static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
     d.rcv_x = rcv; d.rcv_z = rcv + j.nr; d.rcv_i = rcv + 2 * j.nr;
     d.rcv_off = (int *)(mem + o_off);
 
-    // per-cell coefficients (ldz-strided, zero padding):
-    //   c1 = rho v^2 dt, c2 = rho v^2 dt * v^2 dt^2 / 12 (addsa_kernel, acps4.cu:1488)
-    std::vector<float> hc1(N, 0.f), hc2(N, 0.f), hbx(N, 0.f), hbz(N, 0.f);
+    // per-cell coefficients (ldz-strided, zero padding)
     const float dt = j.dt;
+    std::vector<float> hbx(N, 0.f), hbz(N, 0.f), hf1(N, 0.f), hf2(N, 0.f), hf3(N, 0.f);
+    std::vector<float> hq[4];
+    if (q)
+        for (int i = 0; i < 4; i++) hq[i].assign(N, 0.f);
     for (int ix = 0; ix < nxx; ix++)
         for (int iz = 0; iz < nzz; iz++) {
             const size_t a = (size_t)ix * nzz + iz, g = (size_t)ix * ldz + iz;
             const float v = j.vp[a];
-            const float r = j.rho[a] * v * v * dt;
-            hc1[g] = r;
-            hc2[g] = (float)((double)r * (1.0 / 12.0) * v * dt * v * dt);
             hbx[g] = dt * j.bbx[a];
             hbz[g] = dt * j.bbz[a];
+            if (!q) {
+                // c1 = rho v^2 dt, c2 = rho v^2 dt * v^2 dt^2 / 12 (addsa_kernel, acps4.cu:1488)
+                const float r = j.rho[a] * v * v * dt;
+                hf2[g] = r;
+                hf3[g] = (float)((double)r * (1.0 / 12.0) * v * dt * v * dt);
+            } else {
+                // acsglw.cu:1528-1529: p += (rho v^2 dt lam)(total - (dt tao/2) T) - (dt tao) aux
+                const float gm = j.gam[a], lam = j.lam[a], tao = j.tao[a], vwd = j.vwd[a];
+                hf2[g] = j.rho[a] * v * v * dt * lam;
+                hf3[g] = (float)(0.5 * dt * tao);
+                hq[0][g] = dt * tao;
+                // sum_total (acsglw.cu:1232-1240): y2 = 32 gam vwd, c = dt^2 v^2 lam / 12
+                hq[1][g] = gm;
+                hq[2][g] = (float)(32.0 * gm * vwd);
+                const float dtf = dt * dt, vpf = v * v;
+                hq[3][g] = (float)(1.0 / 12.0 * dtf * vpf * lam);
+            }
         }
     std::vector<float> h1x(4 * nxx), h1z(4 * nzz);
     memcpy(&h1x[0], j.iax, 4 * nxx); memcpy(&h1x[nxx], j.ibx, 4 * nxx);
     memcpy(&h1x[2 * nxx], j.hax, 4 * nxx); memcpy(&h1x[3 * nxx], j.hbx, 4 * nxx);
     memcpy(&h1z[0], j.iaz, 4 * nzz); memcpy(&h1z[nzz], j.ibz, 4 * nzz);
     memcpy(&h1z[2 * nzz], j.haz, 4 * nzz); memcpy(&h1z[3 * nzz], j.hbz, 4 * nzz);
-    std::vector<float2> t1, t2, t3, hmx(3 * nxx), hmz(3 * nzz);
-    deriv_table(nxx, j.kxsmr, j.kxsmi, t1);   // v -> p, x: "sm" (acps4.cu:526)
-    deriv_table(nxx, j.kxscr, j.kxsci, t2);   // p -> v, x: "sc" (acps4.cu:594)
-    lap_table(nxx, j.dx, t3);
-    std::copy(t1.begin(), t1.end(), hmx.begin());
-    std::copy(t2.begin(), t2.end(), hmx.begin() + nxx);
-    std::copy(t3.begin(), t3.end(), hmx.begin() + 2 * nxx);
-    deriv_table(nzz, j.kzscr, j.kzsci, t1);   // v -> p, z: "sc" (acps4.cu:527)
-    deriv_table(nzz, j.kzsmr, j.kzsmi, t2);   // p -> v, z: "sm" (acps4.cu:595)
-    lap_table(nzz, j.dz, t3);
-    std::copy(t1.begin(), t1.end(), hmz.begin());
-    std::copy(t2.begin(), t2.end(), hmz.begin() + nzz);
-    std::copy(t3.begin(), t3.end(), hmz.begin() + 2 * nzz);
+    std::vector<float2> hmx(3 * nxx), hmz(3 * nzz);
+    deriv_table(nxx, j.kxsmr, j.kxsmi, &hmx[0]);         // v -> p, x: "sm" (acps4.cu:526)
+    deriv_table(nxx, j.kxscr, j.kxsci, &hmx[nxx]);       // p -> v, x: "sc" (acps4.cu:594)
+    lap_table(nxx, j.dx, &hmx[2 * nxx]);
+    deriv_table(nzz, j.kzscr, j.kzsci, &hmz[0]);         // v -> p, z: "sc" (acps4.cu:527)
+    deriv_table(nzz, j.kzsmr, j.kzsmi, &hmz[nzz]);       // p -> v, z: "sm" (acps4.cu:595)
+    lap_table(nzz, j.dz, &hmz[2 * nzz]);
     // receivers sorted by column, with per-x-tile offsets
     std::vector<int> order(j.nr);
     for (int i = 0; i < j.nr; i++) order[i] = i;
@@ -628,10 +403,13 @@ static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
         hoff[tl] = k;
     }
     hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync((void *)d.c1, hc1.data(), 4 * N, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync((void *)d.c2, hc2.data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync((void *)d.dtbx, hbx.data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync((void *)d.dtbz, hbz.data(), 4 * N, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f2, hf2.data(), 4 * N, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f3, hf3.data(), 4 * N, hipMemcpyHostToDevice, s));
+    if (q)
+        for (int i = 0; i < 4; i++)
+            HIPCHK(hipMemcpyAsync(mem + o_q[i], hq[i].data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(x1, h1x.data(), 4 * h1x.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(z1, h1z.data(), 4 * h1z.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(mx, hmx.data(), 8 * hmx.size(), hipMemcpyHostToDevice, s));
@@ -644,6 +422,27 @@ static int build_job(cq_batch b, const cq_job_desc &j, int cw, JobH &h)
     return CQ_OK;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each XCD with its own L2.  Re-order the work items so that XCD x runs the
+// contiguous chunk [x*chunk, (x+1)*chunk) of the natural order: neighbouring
+// tiles (which share 128-B cache lines along z) then run on the same XCD at
+// about the same time.  Padding items carry job -1 and exit at once.
+static const int kNumXcd = 8;
+
+static std::vector<int2> xcd_order(const std::vector<int2> &items)
+{
+    const size_t n = items.size();
+    const size_t chunk = (n + kNumXcd - 1) / kNumXcd;
+    std::vector<int2> out(chunk * kNumXcd, make_int2(-1, 0));
+    for (size_t b = 0; b < out.size(); b++) {
+        const size_t src = (b % kNumXcd) * chunk + b / kNumXcd;
+        if (src < n) out[b] = items[src];
+    }
+    return out;
+}
+
+static const size_t kLdsMax = 160 * 1024;
+
 extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out)
 {
     if (!ctx || !jobs || njobs < 1 || !out) return set_err(CQ_EINVAL, "cq_batch_create: bad arguments");
@@ -654,101 +453,123 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     }
     std::unique_ptr<cq_batch_s> b(new cq_batch_s());
     b->ctx = ctx;
-    // tile shapes: the largest tiles whose packed lines fit one workgroup's
-    // per-stage capacity for every shot of the batch
-    int tz = 16, cw = 32;
+    // Tile shapes: the largest tiles whose lines fit one workgroup's per-stage
+    // capacity and the LDS for every shot of the batch.
+    int tz = 16, cw = 32, tk = 8;
+    int maxnxx = 0, maxnzz = 0;
     for (int i = 0; i < njobs; i++) {
+        const cq_job_desc &j = jobs[i];
         CqPlan px, pz;
-        if (!make_plan(jobs[i].nxx, px))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has an unsupported prime factor", i, jobs[i].nxx);
-        if (!make_plan(jobs[i].nzz, pz))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has an unsupported prime factor", i, jobs[i].nzz);
-        while (tz > 8 && (tz / 2) * jobs[i].nxx > plan_capacity(px)) tz /= 2;
-        while (cw > 2 && (cw / 2) * jobs[i].nzz > plan_capacity(pz)) cw /= 2;
-        if ((tz / 2) * jobs[i].nxx > plan_capacity(px) || (cw / 2) * jobs[i].nzz > plan_capacity(pz))
-            return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", i, jobs[i].nxx, jobs[i].nzz);
+        if (!make_plan(j.nxx, px))
+            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nxx);
+        if (!make_plan(j.nzz, pz))
+            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nzz);
+        const bool q = j.kind == CQ_KIND_VISCO;
+
+        while (tz > 8 && ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax)) tz /= 2;
+        while (cw > 2 && ((cw / 2) * j.nzz > plan_capacity(pz) ||
+                          (size_t)(1 + cw / 2) * j.nzz * 8 + (q ? (size_t)cw * j.nzz * 4 : 0) > kLdsMax))
+            cw /= 2;
+        if (q)
+            while (tk > 2 && (tk * j.nxx > plan_capacity(px) || (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)) tk /= 2;
+        if ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax ||
+            (cw / 2) * j.nzz > plan_capacity(pz) || (q && (tk * j.nxx > plan_capacity(px) ||
+                                                           (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)))
+            return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", i, j.nxx, j.nzz);
+        maxnxx = std::max(maxnxx, j.nxx);
+        maxnzz = std::max(maxnzz, j.nzz);
     }
     b->tz = tz;
     b->cw = cw;
+    b->tk = tk;
     b->jobs.resize(njobs);
-    int maxnxx = 0, maxnzz = 0;
-    std::vector<int2> irow, icol;
+    std::vector<int2> items[P_COUNT];
     for (int i = 0; i < njobs; i++) {
-        int rc = build_job(b.get(), jobs[i], cw, b->jobs[i]);
+        int rc = build_job(b.get(), jobs[i], b->jobs[i]);
         if (rc) return rc;
         const JobH &h = b->jobs[i];
-        maxnxx = std::max(maxnxx, h.nxx);
-        maxnzz = std::max(maxnzz, h.nzz);
         b->tmax = std::max(b->tmax, h.nt);
-        for (int z0 = 0; z0 < h.nzz; z0 += tz) irow.push_back(make_int2(i, z0));
-        for (int x0 = 0; x0 < h.nxx; x0 += cw) icol.push_back(make_int2(i, x0));
+        if (h.kind == CQ_KIND_ACOUSTIC) {
+            for (int z0 = 0; z0 < h.nzz; z0 += tz) items[P_AROW].push_back(make_int2(i, z0));
+            for (int x0 = 0; x0 < h.nxx; x0 += cw) items[P_ACOL].push_back(make_int2(i, x0));
+        } else {
+            for (int x0 = 0; x0 < h.nxx; x0 += cw) {
+                items[P_QC1].push_back(make_int2(i, x0));
+                items[P_QC2].push_back(make_int2(i, x0));
+                items[P_QC3].push_back(make_int2(i, x0));
+            }
+            for (int k0 = 0; k0 < h.d.nzh; k0 += tk) {
+                items[P_QR1].push_back(make_int2(i, k0));
+                items[P_QR2].push_back(make_int2(i, k0));
+            }
+            for (int z0 = 0; z0 < h.nzz; z0 += tz) items[P_QR3].push_back(make_int2(i, z0));
+        }
     }
-    b->lds_row = (size_t)maxnxx * 8 + (size_t)(tz / 2) * maxnxx * 8 + (size_t)tz * maxnxx * 4;
-    b->lds_col = (size_t)maxnzz * 8 + (size_t)(cw / 2) * maxnzz * 8;
-    if (b->lds_row > 160 * 1024 || b->lds_col > 160 * 1024)
-        return set_err(CQ_EUNSUPPORTED, "window too large for LDS (row %zu B, col %zu B)", b->lds_row, b->lds_col);
+    const size_t lrow = (size_t)(1 + tz / 2) * maxnxx * 8;
+    const size_t lcol = (size_t)(1 + cw / 2) * maxnzz * 8;
+    b->lds[P_AROW] = lrow;
+    b->lds[P_QR3] = lrow;
+    b->lds[P_ACOL] = lcol;
+    b->lds[P_QC1] = lcol;
+    b->lds[P_QC2] = lcol + (size_t)cw * maxnzz * 4;
+    b->lds[P_QC3] = lcol + (size_t)cw * maxnzz * 4;
+    b->lds[P_QR1] = (size_t)(1 + 2 * tk) * maxnxx * 8;
+    b->lds[P_QR2] = (size_t)(1 + tk) * maxnxx * 8;
+    for (int p = 0; p < P_COUNT; p++)
+        if (!items[p].empty() && b->lds[p] > kLdsMax)
+            return set_err(CQ_EUNSUPPORTED, "pass %s needs %zu B of LDS", kPassName[p], b->lds[p]);
     std::vector<JobD> hd(njobs);
     for (int i = 0; i < njobs; i++) hd[i] = b->jobs[i].d;
     HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * njobs));
-    HIPCHK(hipMalloc(&b->ditems_row, sizeof(int2) * irow.size()));
-    HIPCHK(hipMalloc(&b->ditems_col, sizeof(int2) * icol.size()));
     HIPCHK(hipMemcpy(b->djobs, hd.data(), sizeof(JobD) * njobs, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->ditems_row, irow.data(), sizeof(int2) * irow.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->ditems_col, icol.data(), sizeof(int2) * icol.size(), hipMemcpyHostToDevice));
-    b->nrow = (int)irow.size();
-    b->ncol = (int)icol.size();
+    for (int p = 0; p < P_COUNT; p++) {
+        if (items[p].empty()) continue;
+        std::vector<int2> o = xcd_order(items[p]);
+        HIPCHK(hipMalloc(&b->ditems[p], sizeof(int2) * o.size()));
+        HIPCHK(hipMemcpy(b->ditems[p], o.data(), sizeof(int2) * o.size(), hipMemcpyHostToDevice));
+        b->nitems[p] = (int)o.size();
+    }
     *out = b.release();
     return CQ_OK;
 }
 
 template <typename K>
-static hipError_t set_lds(K kernel, size_t bytes)
+static int launch(cq_batch b, K kernel, int pass, int step)
 {
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-static int launch_row(cq_batch b, int step)
-{
-    hipStream_t s = b->ctx->stream;
-    if (b->tz == 16) {
-        HIPCHK(set_lds(k_row_ac<16>, b->lds_row));
-        hipLaunchKernelGGL(k_row_ac<16>, dim3(b->nrow), dim3(CQ_NT), b->lds_row, s, b->djobs, b->ditems_row, step);
-    } else {
-        HIPCHK(set_lds(k_row_ac<8>, b->lds_row));
-        hipLaunchKernelGGL(k_row_ac<8>, dim3(b->nrow), dim3(CQ_NT), b->lds_row, s, b->djobs, b->ditems_row, step);
-    }
+    const size_t lds = b->lds[pass];
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, dim3(b->nitems[pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs, b->ditems[pass],
+                       step);
     HIPCHK(hipGetLastError());
     return CQ_OK;
 }
 
-static int launch_col(cq_batch b, int step)
+#define CQ_DISPATCH_TZ(K, pass)                                                   \
+    (b->tz == 16 ? launch(b, K<16>, pass, step) : launch(b, K<8>, pass, step))
+#define CQ_DISPATCH_CW(K, pass)                                                   \
+    (b->cw == 32   ? launch(b, K<32>, pass, step)                                 \
+     : b->cw == 16 ? launch(b, K<16>, pass, step)                                 \
+     : b->cw == 8  ? launch(b, K<8>, pass, step)                                  \
+     : b->cw == 4  ? launch(b, K<4>, pass, step)                                  \
+                   : launch(b, K<2>, pass, step))
+#define CQ_DISPATCH_TK(K, pass)                                                   \
+    (b->tk == 8 ? launch(b, K<8>, pass, step)                                     \
+     : b->tk == 4 ? launch(b, K<4>, pass, step) : launch(b, K<2>, pass, step))
+
+static int launch_pass(cq_batch b, int pass, int step)
 {
-    hipStream_t s = b->ctx->stream;
-    switch (b->cw) {
-    case 32:
-        HIPCHK(set_lds(k_col_ac<32>, b->lds_col));
-        hipLaunchKernelGGL(k_col_ac<32>, dim3(b->ncol), dim3(CQ_NT), b->lds_col, s, b->djobs, b->ditems_col, step);
-        break;
-    case 16:
-        HIPCHK(set_lds(k_col_ac<16>, b->lds_col));
-        hipLaunchKernelGGL(k_col_ac<16>, dim3(b->ncol), dim3(CQ_NT), b->lds_col, s, b->djobs, b->ditems_col, step);
-        break;
-    case 8:
-        HIPCHK(set_lds(k_col_ac<8>, b->lds_col));
-        hipLaunchKernelGGL(k_col_ac<8>, dim3(b->ncol), dim3(CQ_NT), b->lds_col, s, b->djobs, b->ditems_col, step);
-        break;
-    case 4:
-        HIPCHK(set_lds(k_col_ac<4>, b->lds_col));
-        hipLaunchKernelGGL(k_col_ac<4>, dim3(b->ncol), dim3(CQ_NT), b->lds_col, s, b->djobs, b->ditems_col, step);
-        break;
-    default:
-        HIPCHK(set_lds(k_col_ac<2>, b->lds_col));
-        hipLaunchKernelGGL(k_col_ac<2>, dim3(b->ncol), dim3(CQ_NT), b->lds_col, s, b->djobs, b->ditems_col, step);
-        break;
+    switch (pass) {
+    case P_AROW: return CQ_DISPATCH_TZ(k_row_ac, pass);
+    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, pass);
+    case P_QC1: return CQ_DISPATCH_CW(k_q_c1, pass);
+    case P_QR1: return CQ_DISPATCH_TK(k_q_r1, pass);
+    case P_QC2: return CQ_DISPATCH_CW(k_q_c2, pass);
+    case P_QR2: return CQ_DISPATCH_TK(k_q_r2, pass);
+    case P_QC3: return CQ_DISPATCH_CW(k_q_c3, pass);
+    case P_QR3: return CQ_DISPATCH_TZ(k_q_r3, pass);
     }
-    HIPCHK(hipGetLastError());
-    return CQ_OK;
+    return set_err(CQ_EINVAL, "unknown pass %d", pass);
 }
 
 extern "C" int cq_batch_step(cq_batch b, int nsteps)
@@ -756,24 +577,28 @@ extern "C" int cq_batch_step(cq_batch b, int nsteps)
     if (!b || nsteps < 0) return set_err(CQ_EINVAL, "cq_batch_step: bad arguments");
     HIPCHK(hipSetDevice(b->ctx->device));
     for (int s = 0; s < nsteps && b->t < b->tmax; s++) {
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+        int nev = 0;
         if (b->timing) {
+            hipEvent_t e0;
             HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            HIPCHK(hipEventCreate(&e2));
             HIPCHK(hipEventRecord(e0, b->ctx->stream));
-        }
-        int rc = launch_row(b, b->t);
-        if (rc) return rc;
-        if (b->timing) HIPCHK(hipEventRecord(e1, b->ctx->stream));
-        rc = launch_col(b, b->t);
-        if (rc) return rc;
-        if (b->timing) {
-            HIPCHK(hipEventRecord(e2, b->ctx->stream));
             b->ev.push_back(e0);
-            b->ev.push_back(e1);
-            b->ev.push_back(e2);
+            b->ev_pass.push_back(-1);
         }
+        for (int p = 0; p < P_COUNT; p++) {
+            if (b->nitems[p] == 0) continue;
+            int rc = launch_pass(b, p, b->t);
+            if (rc) return rc;
+            if (b->timing) {
+                hipEvent_t e1;
+                HIPCHK(hipEventCreate(&e1));
+                HIPCHK(hipEventRecord(e1, b->ctx->stream));
+                b->ev.push_back(e1);
+                b->ev_pass.push_back(p);
+                nev++;
+            }
+        }
+        if (b->timing) b->step_nev.push_back(nev);
         b->t++;
     }
     return CQ_OK;
@@ -838,27 +663,75 @@ extern "C" int cq_batch_enable_timing(cq_batch b, int on)
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     for (auto e : b->ev) (void)hipEventDestroy(e);
     b->ev.clear();
+    b->ev_pass.clear();
+    b->step_nev.clear();
     b->timing = on != 0;
     return CQ_OK;
 }
 
+extern "C" int cq_batch_kernel_times(cq_batch b, double *ms, int *calls)
+{
+    if (!b || !ms || !calls) return set_err(CQ_EINVAL, "bad arguments");
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    for (int p = 0; p < P_COUNT; p++) {
+        ms[p] = 0.0;
+        calls[p] = 0;
+    }
+    for (size_t i = 1; i < b->ev.size(); i++) {
+        const int p = b->ev_pass[i];
+        if (p < 0) continue;
+        float a = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, b->ev[i - 1], b->ev[i]));
+        ms[p] += a;
+        calls[p]++;
+    }
+    for (int p = 0; p < P_COUNT; p++)
+        if (calls[p]) ms[p] /= calls[p];
+    return CQ_OK;
+}
+
+extern "C" const char *cq_pass_name(int pass) { return (pass >= 0 && pass < P_COUNT) ? kPassName[pass] : ""; }
+
 extern "C" int cq_batch_pass_times(cq_batch b, double *row_ms, double *col_ms, int *count)
 {
     if (!b || !row_ms || !col_ms || !count) return set_err(CQ_EINVAL, "bad arguments");
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    double sr = 0, sc = 0;
-    int n = 0;
-    for (size_t i = 0; i + 2 < b->ev.size(); i += 3) {
-        float a = 0, c = 0;
-        HIPCHK(hipEventElapsedTime(&a, b->ev[i], b->ev[i + 1]));
-        HIPCHK(hipEventElapsedTime(&c, b->ev[i + 1], b->ev[i + 2]));
-        sr += a;
-        sc += c;
-        n++;
+    double ms[P_COUNT];
+    int calls[P_COUNT];
+    int rc = cq_batch_kernel_times(b, ms, calls);
+    if (rc) return rc;
+    *row_ms = ms[P_AROW];
+    *col_ms = ms[P_ACOL];
+    *count = calls[P_AROW];
+    return CQ_OK;
+}
+
+extern "C" int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *ms_per_copy)
+{
+    if (!ctx || bytes < 1024 || reps < 1 || !ms_per_copy) return set_err(CQ_EINVAL, "cq_copy_calibrate: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const long long n4 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n4 * 16));
+    if (hipMalloc(&b, n4 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return set_err(CQ_ENOMEM, "cq_copy_calibrate: hipMalloc failed");
     }
-    *row_ms = n ? sr / n : 0.0;
-    *col_ms = n ? sc / n : 0.0;
-    *count = n;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a, 0, n4 * 16, ctx->stream));
+    hipLaunchKernelGGL(k_copy_f4, dim3(4096), dim3(256), 0, ctx->stream, a, b, n4);
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_f4, dim3(4096), dim3(256), 0, ctx->stream, a, b, n4);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return CQ_OK;
 }
 

@@ -14,7 +14,7 @@
 #ifndef CQ_NT
 #define CQ_NT 256
 #endif
-#define CQ_MAXP 32       // complex values a thread holds per stage
+#define CQ_MAXP 34       // complex values a thread holds per stage (2 radix-17 butterflies)
 #define CQ_MAXST 12      // max stages per plan
 
 struct CqPlan {

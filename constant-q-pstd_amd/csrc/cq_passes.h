@@ -1,0 +1,660 @@
+// cq_passes.h -- fused time-step passes (HIP, gfx950).
+//
+// Work items are (shot, tile) pairs; one workgroup of CQ_NT threads owns one
+// tile and runs every transform of the pass in LDS.
+//   row tiles  : TZ consecutive depths x all nxx columns, TZ/2 packed complex
+//                lines (real rows 2q and 2q+1 ride as real/imag parts)
+//   col tiles  : CW consecutive columns x all nzz depths, CW/2 packed lines
+//   krow tiles : TK consecutive kz rows of a half spectrum x all nxx columns,
+//                TK complex lines
+// A packed pair of real lines is only ever multiplied by Hermitian
+// multipliers, so the real and imaginary parts never mix.
+#pragma once
+#include "cq_job.h"
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+__device__ __forceinline__ float f4get(const float4 &v, int k)
+{
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void f4set(float4 &v, int k, float s)
+{
+    if (k == 0) v.x = s;
+    else if (k == 1) v.y = s;
+    else if (k == 2) v.z = s;
+    else v.w = s;
+}
+
+__device__ __forceinline__ void load_tw(float2 *tw, const float2 *__restrict__ g, int n)
+{
+    for (int i = threadIdx.x; i < n; i += CQ_NT) tw[i] = g[i];
+}
+
+// ===========================================================================
+// row helpers (real rows, packed)
+// ===========================================================================
+template <int Q>
+__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float *__restrict__ src, int ldz, int z0)
+{
+    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
+        const int ix = e / Q, q = e % Q;
+        const float4 d = ld4(src + (size_t)ix * ldz + z0 + 4 * q);
+        buf[(2 * q) * nxx + ix] = make_float2(d.x, d.y);
+        buf[(2 * q + 1) * nxx + ix] = make_float2(d.z, d.w);
+    }
+}
+
+// x-CPML on dp/dx (in LDS) and the vx update; leaves vx_new in LDS
+// (vx_cal_kernel, sgpstd-qmodeling/acps4.cu:1274-1303)
+template <int Q>
+__device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
+{
+    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml;
+    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
+        const int ix = e / Q, q = e % Q;
+        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        float4 dp = make_float4(a.x, a.y, b.x, b.y);
+        const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
+        if (ix < pml || ix >= nxx - pml) {
+            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+            float *php = J.phix + (size_t)sx * ldz + z0 + 4 * q;
+            const float hb = J.hbx[ix], ha = J.hax[ix];
+            float4 ph = ld4(php);
+            ph.x = hb * ph.x + ha * dp.x;
+            ph.y = hb * ph.y + ha * dp.y;
+            ph.z = hb * ph.z + ha * dp.z;
+            ph.w = hb * ph.w + ha * dp.w;
+            st4(php, ph);
+            dp.x += ph.x; dp.y += ph.y; dp.z += ph.z; dp.w += ph.w;
+        }
+        float4 vn;
+        vn.x = vx.x + bx.x * dp.x;
+        vn.y = vx.y + bx.y * dp.y;
+        vn.z = vx.z + bx.z * dp.z;
+        vn.w = vx.w + bx.w * dp.w;
+        st4(J.vx + g, vn);
+        buf[(2 * q) * nxx + ix] = make_float2(vn.x, vn.y);
+        buf[(2 * q + 1) * nxx + ix] = make_float2(vn.z, vn.w);
+    }
+}
+
+// dvx/dx (in LDS) with its x-CPML memory variable -> dvx for the next step
+// (update_vxx_vzz, acps4.cu:1111-1144)
+template <int Q>
+__device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, int z0)
+{
+    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml;
+    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
+        const int ix = e / Q, q = e % Q;
+        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        float4 dv = make_float4(a.x, a.y, b.x, b.y);
+        if (ix < pml || ix >= nxx - pml) {
+            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+            float *psp = J.psix + (size_t)sx * ldz + z0 + 4 * q;
+            const float ib = J.ibx[ix], ia = J.iax[ix];
+            float4 ps = ld4(psp);
+            ps.x = ib * ps.x + ia * dv.x;
+            ps.y = ib * ps.y + ia * dv.y;
+            ps.z = ib * ps.z + ia * dv.z;
+            ps.w = ib * ps.w + ia * dv.w;
+            st4(psp, ps);
+            dv.x += ps.x; dv.y += ps.y; dv.z += ps.z; dv.w += ps.w;
+        }
+        st4(J.dvx + g, dv);
+    }
+}
+
+// ===========================================================================
+// column helpers (real columns, packed; bufs = float view of the lines)
+// ===========================================================================
+__device__ __forceinline__ float *col_line(float *bufs, int nzz, int c)
+{
+    return bufs + 2 * (c >> 1) * nzz + (c & 1);
+}
+
+template <int CW>
+__device__ __forceinline__ void col_fill(float *bufs, const JobD &J, const float *__restrict__ src, int x0)
+{
+    const int nzz = J.nzz, nq = (nzz + 3) >> 2;
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ix < J.nxx) v = ld4(src + (size_t)ix * J.ldz + 4 * zq);
+        float *line = col_line(bufs, nzz, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
+    }
+}
+
+// gather rec[t][i] from a column tile held as plain floats (src(c, iz))
+template <int CW, typename F>
+__device__ __forceinline__ void col_gather(const JobD &J, int x0, int t, F src)
+{
+    const int tile = x0 / CW;
+    const int r0 = J.rcv_off[tile], r1 = J.rcv_off[tile + 1];
+    for (int r = r0 + threadIdx.x; r < r1; r += CQ_NT)
+        J.rec[(size_t)t * J.nr + J.rcv_i[r]] = src(J.rcv_x[r] - x0, J.rcv_z[r]);
+}
+
+// z-CPML on dp/dz and the vz update; leaves vz_new in LDS
+// (vz_cal_kernel, acps4.cu:1305-1333)
+template <int CW>
+__device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
+{
+    const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const size_t g = (size_t)ix * J.ldz + 4 * zq;
+        float *line = col_line(bufs, nzz, c);
+        const float4 vz = ld4(J.vz + g), bz = ld4(J.dtbz + g);
+        float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz >= nzz) continue;
+            float dp = line[2 * iz];
+            if (iz < pml || iz >= nzz - pml) {
+                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                float *php = J.phiz + (size_t)ix * (2 * pml) + sz;
+                const float ph = J.hbz[iz] * (*php) + J.haz[iz] * dp;
+                *php = ph;
+                dp += ph;
+            }
+            const float v = f4get(vz, k) + f4get(bz, k) * dp;
+            f4set(vn, k, v);
+            line[2 * iz] = v;
+        }
+        st4(J.vz + g, vn);
+    }
+}
+
+// dvz/dz with z-CPML, div = dvx + dvz (update_vxx_vzz + sumpxpz,
+// acps4.cu:1111-1172); stores div and leaves it in LDS
+template <int CW>
+__device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0)
+{
+    const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        float *line = col_line(bufs, nzz, c);
+        if (ix >= J.nxx) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+            continue;
+        }
+        const size_t g = (size_t)ix * J.ldz + 4 * zq;
+        const float4 dvx = ld4(J.dvx + g);
+        float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz >= nzz) continue;
+            float d = line[2 * iz];
+            if (iz < pml || iz >= nzz - pml) {
+                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                float *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
+                const float ps = J.ibz[iz] * (*psp) + J.iaz[iz] * d;
+                *psp = ps;
+                d += ps;
+            }
+            const float s = f4get(dvx, k) + d;
+            f4set(dv, k, s);
+            line[2 * iz] = s;
+        }
+        st4(J.div + g, dv);
+    }
+}
+
+// After a forward z-FFT of packed columns (c0 + i c1): split into the two
+// half spectra G0 = (Z[k] + conj Z[-k])/2, G1 = (Z[k] - conj Z[-k])/(2i),
+// k in [0, nzh), and store them.  With mh != null also multiply Z by mh in
+// place (each thread owns the pair {k, -k}).
+template <int CW>
+__device__ __forceinline__ void col_extract(const JobD &J, float2 *buf, int x0, float2 *__restrict__ S,
+                                            const float2 *__restrict__ mh)
+{
+    constexpr int L = CW / 2;
+    const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
+    for (int e = threadIdx.x; e < L * nzh; e += CQ_NT) {
+        const int l = e / nzh, k = e - l * nzh;
+        const int k2 = k ? nzz - k : 0;
+        float2 *line = buf + l * nzz;
+        const float2 zk = line[k], zm = line[k2];
+        const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        const int ix0 = x0 + 2 * l;
+        if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
+        if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
+        if (mh != nullptr) {
+            line[k] = cmul(zk, mh[k]);
+            if (k2 != k) line[k2] = cmul(zm, mh[k2]);
+        }
+    }
+}
+
+// Inverse of col_extract: rebuild the packed full spectrum W = H0 + i H1 from
+// two Hermitian half spectra (imaginary parts dropped at the self-conjugate
+// bins k = 0 and k = nzz/2, as a complex-to-real transform does).
+template <int CW>
+__device__ __forceinline__ void col_insert(const JobD &J, float2 *buf, int x0, const float2 *__restrict__ S)
+{
+    constexpr int L = CW / 2;
+    const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
+    for (int e = threadIdx.x; e < L * nzh; e += CQ_NT) {
+        const int l = e / nzh, k = e - l * nzh;
+        const int k2 = k ? nzz - k : 0;
+        const int ix0 = x0 + 2 * l;
+        float2 h0 = make_float2(0.f, 0.f), h1 = make_float2(0.f, 0.f);
+        if (ix0 < J.nxx) h0 = S[(size_t)ix0 * ldk + k];
+        if (ix0 + 1 < J.nxx) h1 = S[(size_t)(ix0 + 1) * ldk + k];
+        float2 *line = buf + l * nzz;
+        if (k2 == k) {
+            h0.y = 0.f;
+            h1.y = 0.f;
+        }
+        line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
+        if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
+    }
+}
+
+// ===========================================================================
+// kz-row helpers (half spectra, TK complex lines of length nxx)
+// ===========================================================================
+template <int TK>
+__device__ __forceinline__ void krow_fill(float2 *buf, const JobD &J, const float2 *__restrict__ S, int k0)
+{
+    constexpr int H = TK / 2;
+    const int nxx = J.nxx;
+    for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
+        const int ix = e / H, h = e % H;
+        const float4 v = *reinterpret_cast<const float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
+        buf[(2 * h) * nxx + ix] = make_float2(v.x, v.y);
+        buf[(2 * h + 1) * nxx + ix] = make_float2(v.z, v.w);
+    }
+}
+
+template <int TK>
+__device__ __forceinline__ void krow_store(const float2 *buf, const JobD &J, float2 *__restrict__ S, int k0)
+{
+    constexpr int H = TK / 2;
+    const int nxx = J.nxx;
+    for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
+        const int ix = e / H, h = e % H;
+        const float2 a = buf[(2 * h) * nxx + ix], b = buf[(2 * h + 1) * nxx + ix];
+        *reinterpret_cast<float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
+    }
+}
+
+// Fractional-Laplacian symbols of the viscoacoustic step on the reference's
+// float wavenumber grid (cal_k2d, sgpstd-qmodeling/func.c:550-583):
+//   0: |k|^(1/16) = (kx^2+kz^2)^(1/32)   (k2d2)
+//   1: |k|^2                              (k2d1)
+//   2: |k|^(33/16)                        (k2d3)
+//   3: |k|^1.01 = (kx^2+kz^2)^0.505       (k2da)
+__device__ __forceinline__ float frac_symbol(int which, float k2)
+{
+    if (which == 1) return k2;
+    const float l = log2f(k2);
+    if (which == 3) return exp2f(0.505f * l);
+    const float r = exp2f(l * (1.0f / 32.0f));
+    return which == 0 ? r : k2 * r;
+}
+
+// buf[l][kx] = (save ? copy : buf)[l][kx] * symbol(kx, k0 + l) / N; with
+// save != null the unmultiplied values are first copied there.
+template <int TK>
+__device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *save, const float2 *from, int k0,
+                                          int which)
+{
+    const int nxx = J.nxx;
+    for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
+        const int l = e / nxx, kx = e - l * nxx;
+        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
+        const float fkx = (float)ikx * J.dkx;
+        const float fkz = (float)(k0 + l) * J.dkz;
+        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
+        const float m = frac_symbol(which, k2) * J.invN;
+        float2 v = from ? from[e] : buf[e];
+        if (save) save[e] = v;
+        buf[e] = make_float2(v.x * m, v.y * m);
+    }
+}
+
+// ===========================================================================
+// acoustic passes (sgpstd-qmodeling/acps4.cu:508-645)
+// ===========================================================================
+#define CQ_ITEM_PROLOGUE                      \
+    const int2 it = items[blockIdx.x];        \
+    if (it.x < 0) return;                     \
+    const JobD &J = jobs[it.x];               \
+    const int t = step;                       \
+    if (t >= J.nt) return;
+
+// row pass: L = Lx + Lz, p update + source, dp/dx -> vx, dvx/dx -> dvx
+template <int TZ>
+__global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                  int step)
+{
+    constexpr int L = TZ / 2, Q = TZ / 4;
+    CQ_ITEM_PROLOGUE
+    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nxx;
+    load_tw(tw, J.twx, nxx);
+    row_fill<Q>(buf, nxx, J.div, ldz, z0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mx2);
+    cq_fft_inv(J.px, buf, nxx, L, tw);
+
+    // p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498)
+    const float src = J.rik[t] * J.src_scale;
+    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
+        const int ix = e / Q, q = e % Q;
+        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g), lz = ld4(J.lz + g),
+                     d = ld4(J.div + g);
+        float4 pn;
+        pn.x = p.x + c1.x * d.x - c2.x * (a.x + lz.x);
+        pn.y = p.y + c1.y * d.y - c2.y * (a.y + lz.y);
+        pn.z = p.z + c1.z * d.z - c2.z * (b.x + lz.z);
+        pn.w = p.w + c1.w * d.w - c2.w * (b.y + lz.w);
+        if (ix == J.xsg) {
+            const int dz = J.zsg - (z0 + 4 * q);
+            if (dz >= 0 && dz < 4) f4set(pn, dz, f4get(pn, dz) + src);
+        }
+        st4(J.p + g, pn);
+        buf[(2 * q) * nxx + ix] = make_float2(pn.x, pn.y);
+        buf[(2 * q + 1) * nxx + ix] = make_float2(pn.z, pn.w);
+    }
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
+    cq_fft_inv(J.px, buf, nxx, L, tw);
+    row_vx_phase<Q>(J, buf, z0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
+    cq_fft_inv(J.px, buf, nxx, L, tw);
+    row_dvx_phase<Q>(J, buf, z0);
+}
+
+// column pass: gather, dp/dz -> vz, dvz/dz -> div, Lz = d2/dz2 div
+template <int CW>
+__global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                  int step)
+{
+    constexpr int L = CW / 2;
+    CQ_ITEM_PROLOGUE
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nzz;
+    float *bufs = reinterpret_cast<float *>(buf);
+    load_tw(tw, J.twz, nzz);
+    col_fill<CW>(bufs, J, J.p, x0);
+    __syncthreads();
+    // receiver gather of the updated p (acps4.cu:611-621)
+    col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzp);
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_vz_phase<CW>(J, bufs, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_div_phase<CW>(J, bufs, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mz2);
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const float *line = col_line(bufs, nzz, c);
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (4 * zq + k < nzz) f4set(o, k, line[2 * (4 * zq + k)]);
+        st4(J.lz + (size_t)ix * J.ldz + 4 * zq, o);
+    }
+}
+
+// ===========================================================================
+// viscoacoustic passes (sgpstd-qmodeling/acsglw.cu:511-683)
+// One step = C1 R1 C2 R2 C3 R3:
+//   C1 (cols) : dp/dz -> vz, dvz/dz -> div; half spectra F_z p (s2), F_z div (s1)
+//   R1 (kz)   : F_x, x3 symbols -> A (s1), K (s3), B (s4); F_x p x |k|^1.01 -> aux (s2)
+//   C2 (cols) : F_z^-1 A, K, B; total (sum_total, acsglw.cu:1213-1245); F_z total (s1)
+//   R2 (kz)   : F_x total x |k|^1.01 -> T (s1)
+//   C3 (cols) : F_z^-1 T, aux; p update + source (acsglw.cu:1500-1537); gather
+//   R3 (rows) : dp/dx -> vx, dvx/dx -> dvx
+// ===========================================================================
+template <int CW>
+__global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    constexpr int L = CW / 2;
+    CQ_ITEM_PROLOGUE
+    const int nzz = J.nzz, x0 = it.y;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nzz;
+    float *bufs = reinterpret_cast<float *>(buf);
+    load_tw(tw, J.twz, nzz);
+    col_fill<CW>(bufs, J, J.p, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    col_extract<CW>(J, buf, x0, J.s2, J.mzp);
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_vz_phase<CW>(J, bufs, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_div_phase<CW>(J, bufs, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    col_extract<CW>(J, buf, x0, J.s1, nullptr);
+}
+
+template <int TK>
+__global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    CQ_ITEM_PROLOGUE
+    const int nxx = J.nxx, k0 = it.y;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nxx, *sav = buf + TK * nxx;
+    load_tw(tw, J.twx, nxx);
+    krow_fill<TK>(buf, J, J.s1, k0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
+    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566)
+    float2 *outs[3] = {J.s1, J.s3, J.s4};
+    for (int w = 0; w < 3; w++) {
+        krow_mult<TK>(J, buf, w == 0 ? sav : nullptr, w == 0 ? nullptr : sav, k0, w);
+        __syncthreads();
+        cq_fft_inv(J.px, buf, nxx, TK, tw);
+        krow_store<TK>(buf, J, outs[w], k0);
+        __syncthreads();
+    }
+    // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
+    krow_fill<TK>(buf, J, J.s2, k0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
+    krow_mult<TK>(J, buf, nullptr, nullptr, k0, 3);
+    __syncthreads();
+    cq_fft_inv(J.px, buf, nxx, TK, tw);
+    krow_store<TK>(buf, J, J.s2, k0);
+}
+
+template <int CW>
+__global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    constexpr int L = CW / 2;
+    CQ_ITEM_PROLOGUE
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nzz;
+    float *bufs = reinterpret_cast<float *>(buf);
+    float *acc = reinterpret_cast<float *>(buf + L * nzz);
+    load_tw(tw, J.twz, nzz);
+    // total = y1 div + y2 A - (dt^2 v^2 lam/12)(y3 K + y4 B), with
+    // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2
+    col_insert<CW>(J, buf, x0, J.s3);   // K
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const float *line = col_line(bufs, nzz, c);
+        const float4 g = ld4(J.qg + (size_t)ix * ldz + 4 * zq);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz < nzz) acc[c * nzz + iz] = (1.0f - 64.0f * f4get(g, k)) * line[2 * iz];
+        }
+    }
+    __syncthreads();
+    col_insert<CW>(J, buf, x0, J.s4);   // B
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const float *line = col_line(bufs, nzz, c);
+        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const float4 y2 = ld4(J.qy2 + g), cc = ld4(J.qcc + g);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz < nzz)
+                acc[c * nzz + iz] = f4get(cc, k) * (acc[c * nzz + iz] + 2.0f * f4get(y2, k) * line[2 * iz]);
+        }
+    }
+    __syncthreads();
+    col_insert<CW>(J, buf, x0, J.s1);   // A
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        float *line = col_line(bufs, nzz, c);
+        if (ix >= J.nxx) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+            continue;
+        }
+        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g);
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz >= nzz) continue;
+            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * iz] -
+                             acc[c * nzz + iz];
+            f4set(o, k, tv);
+            line[2 * iz] = tv;
+        }
+        st4(J.tot + g, o);
+    }
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    col_extract<CW>(J, buf, x0, J.s1, nullptr);
+}
+
+template <int TK>
+__global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    CQ_ITEM_PROLOGUE
+    const int nxx = J.nxx, k0 = it.y;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nxx;
+    load_tw(tw, J.twx, nxx);
+    krow_fill<TK>(buf, J, J.s1, k0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
+    krow_mult<TK>(J, buf, nullptr, nullptr, k0, 3);   // T = |k|^1.01 total (acsglw.cu:578-588)
+    __syncthreads();
+    cq_fft_inv(J.px, buf, nxx, TK, tw);
+    krow_store<TK>(buf, J, J.s1, k0);
+}
+
+template <int CW>
+__global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    constexpr int L = CW / 2;
+    CQ_ITEM_PROLOGUE
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nzz;
+    float *bufs = reinterpret_cast<float *>(buf);
+    float *acc = reinterpret_cast<float *>(buf + L * nzz);
+    load_tw(tw, J.twz, nzz);
+    col_insert<CW>(J, buf, x0, J.s1);   // T
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq;
+        const float *line = col_line(bufs, nzz, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz < nzz) acc[c * nzz + iz] = line[2 * iz];
+        }
+    }
+    __syncthreads();
+    col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    // p += rho v^2 dt lam (total - dt tao T / 2) - dt tao aux; source
+    const float src = J.rik[t] * J.src_scale;
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const float *line = col_line(bufs, nzz, c);
+        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
+                     qe = ld4(J.qe + g);
+        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int iz = 4 * zq + k;
+            if (iz >= nzz) continue;
+            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - f4get(qb, k) * acc[c * nzz + iz]) -
+                      f4get(qe, k) * line[2 * iz];
+            if (ix == J.xsg && iz == J.zsg) v += src;
+            f4set(pn, k, v);
+            acc[c * nzz + iz] = v;
+        }
+        st4(J.p + g, pn);
+    }
+    __syncthreads();
+    col_gather<CW>(J, x0, t, [&](int c, int z) { return acc[c * nzz + z]; });
+}
+
+template <int TZ>
+__global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                int step)
+{
+    constexpr int L = TZ / 2, Q = TZ / 4;
+    CQ_ITEM_PROLOGUE
+    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y;
+    extern __shared__ float2 lds[];
+    float2 *tw = lds, *buf = lds + nxx;
+    load_tw(tw, J.twx, nxx);
+    row_fill<Q>(buf, nxx, J.p, ldz, z0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
+    cq_fft_inv(J.px, buf, nxx, L, tw);
+    row_vx_phase<Q>(J, buf, z0);
+    __syncthreads();
+    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
+    cq_fft_inv(J.px, buf, nxx, L, tw);
+    row_dvx_phase<Q>(J, buf, z0);
+}

@@ -99,10 +99,21 @@ int cq_batch_destroy(cq_batch b);
  * reference's per-step structure, acps4.cu:508-621).  rec receives nt*nr. */
 int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec);
 
-/* Average device time (ms) of the last `n` launches of the named pass
- * ("row" or "col") recorded with HIP events when timing is enabled. */
+/* Per-pass device timing with HIP events on the engine's stream.  While
+ * enabled, every launch of cq_batch_step is bracketed by events; passes are
+ * numbered 0..7 (cq_pass_name gives the kernel name): acoustic row, acoustic
+ * column, then the six viscoacoustic passes C1 R1 C2 R2 C3 R3. */
 int cq_batch_enable_timing(cq_batch b, int on);
+/* Average ms per launch of each pass (arrays of 8) and launch counts. */
+int cq_batch_kernel_times(cq_batch b, double *ms, int *calls);
+const char *cq_pass_name(int pass);
+/* Acoustic row / column averages (passes 0 and 1). */
 int cq_batch_pass_times(cq_batch b, double *row_ms, double *col_ms, int *count);
+
+/* Measurement helper (no counterpart in the reference): times `reps` runs of
+ * a streaming device copy of `bytes` bytes (read + write) to calibrate HBM
+ * byte counters and achievable bandwidth. */
+int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
