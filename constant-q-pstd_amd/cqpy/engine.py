@@ -27,6 +27,7 @@ class JobDesc(C.Structure):
         ("dx", C.c_float), ("dz", C.c_float),
         ("rik", _F), ("xsg", C.c_int), ("zsg", C.c_int),
         ("nr", C.c_int), ("gxg", _I), ("gzg", _I),
+        ("obs", _F), ("snaps", C.c_int), ("etas", C.c_float), ("etar", C.c_float),
     ]
 
 
@@ -34,7 +35,7 @@ SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_creat
            "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",
            "cq_batch_read_p", "cq_batch_time", "cq_batch_bytes", "cq_batch_destroy",
            "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times",
-           "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name"]
+           "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image"]
 
 _lib = None
 
@@ -89,6 +90,13 @@ def make_desc(job, keep):
     keep += [gx, gz]
     d.xsg, d.zsg, d.nr = int(job["xsg"]), int(job["zsg"]), gx.size
     d.gxg, d.gzg = gx.ctypes.data_as(_I), gz.ctypes.data_as(_I)
+    if job.get("obs") is not None:
+        ob = np.ascontiguousarray(job["obs"], np.float32)
+        keep.append(ob)
+        d.obs = ob.ctypes.data_as(_F)
+    d.snaps = int(job.get("snaps", 0))
+    d.etas = float(job.get("etas", 0.0))
+    d.etar = float(job.get("etar", 0.0))
     return d
 
 
@@ -163,6 +171,16 @@ class Batch:
         out = np.zeros(int(j["nxx"]) * int(j["nzz"]), np.float32)
         _chk(lib().cq_batch_read_p(self.h, C.c_int(i), out.ctypes.data_as(_F)))
         return out
+
+    def image(self, i):
+        """Raw (img, ssg) of RTM shot i, each nx*nz over the interior."""
+        j = self.jobs[i]
+        n = (int(j["nxx"]) - 2 * int(j["pml"])) * (int(j["nzz"]) - 2 * int(j["pml"]))
+        img = np.zeros(n, np.float32)
+        ssg = np.zeros(n, np.float32)
+        _chk(lib().cq_batch_read_image(self.h, C.c_int(i), img.ctypes.data_as(_F),
+                                       ssg.ctypes.data_as(_F)))
+        return img, ssg
 
     def enable_timing(self, on=True):
         _chk(lib().cq_batch_enable_timing(self.h, C.c_int(1 if on else 0)))

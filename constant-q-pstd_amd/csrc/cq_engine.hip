@@ -35,6 +35,7 @@
 
 #include "cq.h"
 #include "cq_passes.h"
+#include "cq_revolve.h"
 
 #define CQ_VERSION "cq-pstd-mi355x 0.2"
 
@@ -179,11 +180,65 @@ struct Arena {
     }
 };
 
+// Per-shot device memory.  The state of a wavefield (what a revolve
+// checkpoint must hold) is one contiguous block, so a takeshot or restore is a
+// single device copy:  p, vx, vz, div, X, Y, psix, phix, psiz, phiz with
+// X = lz (acoustic) or dvz (viscoacoustic) and Y = dvx (acoustic) or the kz
+// half spectrum of p (viscoacoustic).
+struct StateLayout {
+    size_t p, vx, vz, div, x, y, psix, phix, psiz, phiz, bytes;
+};
+
+static StateLayout state_layout(int nxx, int ldz, int ldk, int pml, bool q)
+{
+    Arena A;
+    StateLayout s;
+    const size_t N = (size_t)nxx * ldz;
+    s.p = A.take<float>(N);
+    s.vx = A.take<float>(N);
+    s.vz = A.take<float>(N);
+    s.div = A.take<float>(N);
+    s.x = A.take<float>(N);
+    s.y = q ? A.take<float2>((size_t)nxx * ldk) : A.take<float>(N);
+    s.psix = A.take<float>((size_t)2 * pml * ldz);
+    s.phix = A.take<float>((size_t)2 * pml * ldz);
+    s.psiz = A.take<float>((size_t)2 * pml * nxx);
+    s.phiz = A.take<float>((size_t)2 * pml * nxx);
+    s.bytes = (A.take<char>(0) + 255) & ~(size_t)255;
+    return s;
+}
+
+static void bind_state(JobD &d, char *base, const StateLayout &s, bool q)
+{
+    d.p = (float *)(base + s.p);
+    d.vx = (float *)(base + s.vx);
+    d.vz = (float *)(base + s.vz);
+    d.div = (float *)(base + s.div);
+    if (q) {
+        d.dvz = (float *)(base + s.x);
+        d.s2 = (float2 *)(base + s.y);
+    } else {
+        d.lz = (float *)(base + s.x);
+        d.dvx = (float *)(base + s.y);
+    }
+    d.psix = (float *)(base + s.psix);
+    d.phix = (float *)(base + s.phix);
+    d.psiz = (float *)(base + s.psiz);
+    d.phiz = (float *)(base + s.phiz);
+}
+
 struct JobH {
-    JobD d;
+    JobD d[2];              // role 0: source / modeling wavefield; role 1: receiver wavefield (RTM)
+    int nroles = 1;
     void *mem = nullptr;
     size_t bytes = 0;
-    int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0, kind = 0;
+    char *state[2] = {nullptr, nullptr};
+    char *ckpt = nullptr;   // snaps state blocks
+    size_t state_bytes = 0;
+    int snaps = 0;
+    int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0, kind = 0, nx = 0, nz = 0;
+    size_t src_cell = 0;    // xsg * ldz + zsg
+    float src0 = 0.f;       // rik[0] * dt * 1000 (checkpoint-0 quirk)
 };
 
 // One launch of the per-step sequence.
@@ -194,21 +249,24 @@ static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_
 struct cq_batch_s {
     cq_ctx ctx = nullptr;
     std::vector<JobH> jobs;
-    JobD *djobs = nullptr;
-    int2 *ditems[P_COUNT] = {};
-    int nitems[P_COUNT] = {};
+    bool rtm = false;
+    JobD *djobs = nullptr;                 // [role][job]
+    int2 *ditems[2][P_COUNT] = {};
+    int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, tk = 8;
     int t = 0, tmax = 0;
+    // RTM schedule state
+    int snaps = 0, rtm_done = 0, rv_info = 0;
     bool timing = false;
-    std::vector<hipEvent_t> ev;        // per step: start + one per launch
-    std::vector<int> ev_pass;          // pass id of each event after the start
-    std::vector<int> step_nev;         // launches recorded per step
+    std::vector<hipEvent_t> ev;            // start of step, then one per launch
+    std::vector<int> ev_pass;
     ~cq_batch_s()
     {
         for (auto &j : jobs) (void)hipFree(j.mem);
         (void)hipFree(djobs);
-        for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[i]);
+        for (int r = 0; r < 2; r++)
+            for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
@@ -244,18 +302,21 @@ extern "C" int cq_close(cq_ctx ctx)
     return CQ_OK;
 }
 
+static bool is_q(int kind) { return kind == CQ_KIND_VISCO || kind == CQ_KIND_QRTM; }
+static bool is_rtm(int kind) { return kind == CQ_KIND_RTM || kind == CQ_KIND_QRTM; }
+
 static int validate(const cq_job_desc &j, int idx)
 {
-    if (j.kind != CQ_KIND_ACOUSTIC && j.kind != CQ_KIND_VISCO)
-        return set_err(CQ_EUNSUPPORTED, "job %d: kind %d is not a modeling kind", idx, j.kind);
-    if (j.nxx < 8 || j.nzz < 8 || j.pml < 2 || 2 * j.pml > j.nxx || 2 * j.pml > j.nzz)
+    if (j.kind < CQ_KIND_ACOUSTIC || j.kind > CQ_KIND_QRTM)
+        return set_err(CQ_EINVAL, "job %d: unknown kind %d", idx, j.kind);
+    if (j.nxx < 8 || j.nzz < 8 || j.pml < 2 || 2 * j.pml >= j.nxx || 2 * j.pml >= j.nzz)
         return set_err(CQ_EINVAL, "job %d: bad window %dx%d pml %d", idx, j.nxx, j.nzz, j.pml);
     if (j.nt < 1 || !(j.dt > 0.f)) return set_err(CQ_EINVAL, "job %d: bad nt/dt", idx);
     if (!j.vp || !j.rho || !j.bbx || !j.bbz || !j.rik || !j.iax || !j.ibx || !j.hax || !j.hbx ||
         !j.iaz || !j.ibz || !j.haz || !j.hbz || !j.kxsmr || !j.kxsmi || !j.kxscr || !j.kxsci ||
         !j.kzsmr || !j.kzsmi || !j.kzscr || !j.kzsci)
         return set_err(CQ_EINVAL, "job %d: missing input array", idx);
-    if (j.kind == CQ_KIND_VISCO && (!j.gam || !j.lam || !j.vwd || !j.tao))
+    if (is_q(j.kind) && (!j.gam || !j.lam || !j.vwd || !j.tao))
         return set_err(CQ_EINVAL, "job %d: viscoacoustic job without gam/lam/vwd/tao", idx);
     if (j.xsg < 0 || j.xsg >= j.nxx || j.zsg < 0 || j.zsg >= j.nzz)
         return set_err(CQ_EINVAL, "job %d: source index outside the window", idx);
@@ -265,90 +326,158 @@ static int validate(const cq_job_desc &j, int idx)
         if (j.gxg[i] < 0 || j.gxg[i] >= j.nxx || j.gzg[i] < 0 || j.gzg[i] >= j.nzz)
             return set_err(CQ_EINVAL, "job %d: receiver %d outside the window", idx, i);
     if (!(j.dx > 0.f) || !(j.dz > 0.f)) return set_err(CQ_EINVAL, "job %d: bad dx/dz", idx);
+    if (is_rtm(j.kind)) {
+        // revolve limits checkup = 64 (src.c:176) and snaps >= 1
+        if (j.snaps < 1 || j.snaps > 64) return set_err(CQ_EINVAL, "job %d: snaps %d outside 1..64", idx, j.snaps);
+        if (j.nr > 0 && !j.obs) return set_err(CQ_EINVAL, "job %d: RTM job without observed data", idx);
+    }
     return CQ_OK;
+}
+
+// CSR of items sorted by key into tiles of width `tw` (ntile + 1 offsets)
+static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector<int> &order, std::vector<int> &off)
+{
+    const int n = (int)key.size();
+    order.resize(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key[a] < key[c]; });
+    off.assign(ntile + 1, 0);
+    for (int tl = 0, k = 0; tl <= ntile; tl++) {
+        while (k < n && key[order[k]] < tl * tw) k++;
+        off[tl] = k;
+    }
 }
 
 static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
 {
     cq_ctx ctx = b->ctx;
-    const int cw = b->cw, tk = b->tk;
-    const bool q = j.kind == CQ_KIND_VISCO;
+    const int cw = b->cw, tk = b->tk, tz = b->tz;
+    const bool q = is_q(j.kind), rtm = is_rtm(j.kind);
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = (nzz + 31) & ~31;
     const int nzh = nzz / 2 + 1;
     const int ldk = (nzh + tk - 1) / tk * tk;
+    const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
     const size_t N = (size_t)nxx * ldz;
-    JobD &d = h.d;
-    memset(&d, 0, sizeof d);
-    d.kind = j.kind; d.nxx = nxx; d.nzz = nzz; d.ldz = ldz; d.pml = pml; d.nt = j.nt;
-    d.xsg = j.xsg; d.zsg = j.zsg; d.nr = j.nr; d.nzh = nzh; d.ldk = ldk;
-    d.src_scale = j.dt * 1000.0f;
-    d.dkx = k_step(nxx, j.dx);
-    d.dkz = k_step(nzz, j.dz);
-    d.invN = (float)(1.0 / ((double)nxx * nzz));
-    int rc;
-    if ((rc = get_plan(ctx, nxx, d.px, d.twx)) != CQ_OK) return rc;
-    if ((rc = get_plan(ctx, nzz, d.pz, d.twz)) != CQ_OK) return rc;
+    const int ntile_x = (nxx + cw - 1) / cw, ntile_z = (nzz + tz - 1) / tz;
+    const int nrole = rtm ? 2 : 1;
+    const int snaps = rtm ? j.snaps : 0;
 
-    const int ntile_x = (nxx + cw - 1) / cw;
+    JobD base;
+    memset(&base, 0, sizeof base);
+    base.kind = j.kind; base.nxx = nxx; base.nzz = nzz; base.ldz = ldz; base.pml = pml; base.nt = j.nt;
+    base.xsg = j.xsg; base.zsg = j.zsg; base.nr = j.nr; base.nzh = nzh; base.ldk = ldk;
+    base.nx = nx; base.nz = nz;
+    base.src_scale = j.dt * 1000.0f;
+    base.dt = j.dt;
+    base.dkx = k_step(nxx, j.dx);
+    base.dkz = k_step(nzz, j.dz);
+    base.invN = (float)(1.0 / ((double)nxx * nzz));
+    base.qsign = (j.kind == CQ_KIND_QRTM) ? -1.0f : 1.0f;
+    base.stab = (j.kind == CQ_KIND_QRTM) ? 1 : 0;
+    base.gather = rtm ? 0 : 1;
+    int rc;
+    if ((rc = get_plan(ctx, nxx, base.px, base.twx)) != CQ_OK) return rc;
+    if ((rc = get_plan(ctx, nzz, base.pz, base.twz)) != CQ_OK) return rc;
+
+    const StateLayout sl = state_layout(nxx, ldz, ldk, pml, q);
     Arena A;
-    const size_t o_p = A.take<float>(N), o_vx = A.take<float>(N), o_vz = A.take<float>(N);
-    const size_t o_div = A.take<float>(N), o_dvx = A.take<float>(N);
+    size_t o_state[2] = {0, 0};
+    for (int r = 0; r < nrole; r++) o_state[r] = A.take<char>(sl.bytes);
+    const size_t o_ckpt = A.take<char>(sl.bytes * (size_t)snaps);
     const size_t o_bx = A.take<float>(N), o_bz = A.take<float>(N);
-    // acoustic: lz, c1, c2 ; viscoacoustic: tot, qa, qb, qe, qg, qy2, qcc, 4 half spectra
-    const size_t o_f1 = A.take<float>(N), o_f2 = A.take<float>(N), o_f3 = A.take<float>(N);
-    size_t o_q[4] = {0, 0, 0, 0}, o_s[4] = {0, 0, 0, 0};
+    const size_t o_f2 = A.take<float>(N), o_f3 = A.take<float>(N);
+    size_t o_q[4] = {0, 0, 0, 0}, o_tot = 0, o_s1 = 0, o_s3 = 0, o_s4 = 0;
     if (q) {
         for (int i = 0; i < 4; i++) o_q[i] = A.take<float>(N);
-        for (int i = 0; i < 4; i++) o_s[i] = A.take<float2>((size_t)nxx * ldk);
+        o_tot = A.take<float>(N);
+        o_s1 = A.take<float2>((size_t)nxx * ldk);
+        o_s3 = A.take<float2>((size_t)nxx * ldk);
+        o_s4 = A.take<float2>((size_t)nxx * ldk);
     }
-    const size_t o_psix = A.take<float>((size_t)2 * pml * ldz), o_phix = A.take<float>((size_t)2 * pml * ldz);
-    const size_t o_psiz = A.take<float>((size_t)2 * pml * nxx), o_phiz = A.take<float>((size_t)2 * pml * nxx);
     const size_t o_1x = A.take<float>((size_t)4 * nxx), o_1z = A.take<float>((size_t)4 * nzz);
     const size_t o_mx = A.take<float2>((size_t)3 * nxx), o_mz = A.take<float2>((size_t)3 * nzz);
     const size_t o_rik = A.take<float>(j.nt);
-    const size_t o_rec = A.take<float>((size_t)j.nt * std::max(j.nr, 1));
+    const size_t o_rec = A.take<float>(rtm ? 1 : (size_t)j.nt * std::max(j.nr, 1));
     const size_t o_rcv = A.take<int>((size_t)3 * std::max(j.nr, 1));
     const size_t o_off = A.take<int>((size_t)ntile_x + 1);
+    size_t o_obs = 0, o_img = 0, o_ssg = 0, o_inj = 0, o_ioff = 0;
+    const int ntile_inj = q ? ntile_x : ntile_z;
+    if (rtm) {
+        o_obs = A.take<float>((size_t)j.nt * std::max(j.nr, 1));
+        o_img = A.take<float>((size_t)nx * nz);
+        o_ssg = A.take<float>((size_t)nx * nz);
+        o_inj = A.take<int>((size_t)3 * std::max(j.nr, 1));
+        o_ioff = A.take<int>((size_t)ntile_inj + 1);
+    }
     const size_t total = A.take<char>(0);
     char *mem = nullptr;
     hipError_t e = hipMalloc(&mem, total);
     if (e != hipSuccess) return set_err(CQ_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e));
     h.mem = mem;
     h.bytes = total;
-    h.nt = j.nt; h.nr = j.nr; h.nxx = nxx; h.nzz = nzz; h.ldz = ldz; h.kind = j.kind;
+    h.nroles = nrole;
+    h.state_bytes = sl.bytes;
+    h.snaps = snaps;
+    h.ckpt = mem + o_ckpt;
+    h.nt = j.nt; h.nr = j.nr; h.nxx = nxx; h.nzz = nzz; h.ldz = ldz; h.kind = j.kind; h.nx = nx; h.nz = nz;
+    h.src_cell = (size_t)j.xsg * ldz + j.zsg;
+    h.src0 = j.rik[0] * j.dt * 1000.0f;
     HIPCHK(hipMemsetAsync(mem, 0, total, ctx->stream));
 
-    d.p = (float *)(mem + o_p); d.vx = (float *)(mem + o_vx); d.vz = (float *)(mem + o_vz);
-    d.div = (float *)(mem + o_div); d.dvx = (float *)(mem + o_dvx);
-    d.dtbx = (float *)(mem + o_bx); d.dtbz = (float *)(mem + o_bz);
-    float *f1 = (float *)(mem + o_f1), *f2 = (float *)(mem + o_f2), *f3 = (float *)(mem + o_f3);
+    base.dtbx = (float *)(mem + o_bx); base.dtbz = (float *)(mem + o_bz);
+    float *f2 = (float *)(mem + o_f2), *f3 = (float *)(mem + o_f3);
     if (!q) {
-        d.lz = f1; d.c1 = f2; d.c2 = f3;
+        base.c1 = f2; base.c2 = f3;
     } else {
-        d.tot = f1; d.qa = f2; d.qb = f3;
-        d.qe = (float *)(mem + o_q[0]); d.qg = (float *)(mem + o_q[1]);
-        d.qy2 = (float *)(mem + o_q[2]); d.qcc = (float *)(mem + o_q[3]);
-        d.s1 = (float2 *)(mem + o_s[0]); d.s2 = (float2 *)(mem + o_s[1]);
-        d.s3 = (float2 *)(mem + o_s[2]); d.s4 = (float2 *)(mem + o_s[3]);
+        base.qa = f2; base.qb = f3;
+        base.qe = (float *)(mem + o_q[0]); base.qg = (float *)(mem + o_q[1]);
+        base.qy2 = (float *)(mem + o_q[2]); base.qcc = (float *)(mem + o_q[3]);
+        base.tot = (float *)(mem + o_tot);
+        base.s1 = (float2 *)(mem + o_s1); base.s3 = (float2 *)(mem + o_s3); base.s4 = (float2 *)(mem + o_s4);
     }
-    d.psix = (float *)(mem + o_psix); d.phix = (float *)(mem + o_phix);
-    d.psiz = (float *)(mem + o_psiz); d.phiz = (float *)(mem + o_phiz);
     float *x1 = (float *)(mem + o_1x), *z1 = (float *)(mem + o_1z);
-    d.iax = x1; d.ibx = x1 + nxx; d.hax = x1 + 2 * nxx; d.hbx = x1 + 3 * nxx;
-    d.iaz = z1; d.ibz = z1 + nzz; d.haz = z1 + 2 * nzz; d.hbz = z1 + 3 * nzz;
+    base.iax = x1; base.ibx = x1 + nxx; base.hax = x1 + 2 * nxx; base.hbx = x1 + 3 * nxx;
+    base.iaz = z1; base.ibz = z1 + nzz; base.haz = z1 + 2 * nzz; base.hbz = z1 + 3 * nzz;
     float2 *mx = (float2 *)(mem + o_mx), *mz = (float2 *)(mem + o_mz);
-    d.mxm = mx; d.mxp = mx + nxx; d.mx2 = mx + 2 * nxx;
-    d.mzm = mz; d.mzp = mz + nzz; d.mz2 = mz + 2 * nzz;
-    d.rik = (float *)(mem + o_rik);
-    d.rec = (float *)(mem + o_rec);
+    base.mxm = mx; base.mxp = mx + nxx; base.mx2 = mx + 2 * nxx;
+    base.mzm = mz; base.mzp = mz + nzz; base.mz2 = mz + 2 * nzz;
+    base.rik = (float *)(mem + o_rik);
+    base.rec = (float *)(mem + o_rec);
     int *rcv = (int *)(mem + o_rcv);
-    d.rcv_x = rcv; d.rcv_z = rcv + j.nr; d.rcv_i = rcv + 2 * j.nr;
-    d.rcv_off = (int *)(mem + o_off);
+    base.rcv_x = rcv; base.rcv_z = rcv + j.nr; base.rcv_i = rcv + 2 * j.nr;
+    base.rcv_off = (int *)(mem + o_off);
+    if (rtm) {
+        base.obs = (float *)(mem + o_obs);
+        base.img = (float *)(mem + o_img);
+        base.ssg = (float *)(mem + o_ssg);
+        int *inj = (int *)(mem + o_inj);
+        base.inj_x = inj; base.inj_z = inj + j.nr; base.inj_i = inj + 2 * j.nr;
+        base.inj_off = (int *)(mem + o_ioff);
+        // stabiliser: tao_c = max tao over the padded window (sgpstd-qrtm/acsglw.cu:354-356)
+        if (q) {
+            float mx_ = 0.0f;
+            for (size_t a = 0; a < (size_t)nxx * nzz; a++)
+                if (j.tao[a] >= mx_) mx_ = j.tao[a];
+            base.taoc = mx_;
+        }
+    }
+    for (int r = 0; r < nrole; r++) {
+        h.d[r] = base;
+        h.d[r].role = r;
+        h.state[r] = mem + o_state[r];
+        bind_state(h.d[r], h.state[r], sl, q);
+        if (r == 1) {
+            h.d[r].psrc = h.d[0].p;
+            h.d[r].eta = j.etar;
+        } else {
+            h.d[r].eta = j.etas;
+        }
+    }
 
     // per-cell coefficients (ldz-strided, zero padding)
     const float dt = j.dt;
-    std::vector<float> hbx(N, 0.f), hbz(N, 0.f), hf1(N, 0.f), hf2(N, 0.f), hf3(N, 0.f);
+    std::vector<float> hbx(N, 0.f), hbz(N, 0.f), hf2(N, 0.f), hf3(N, 0.f);
     std::vector<float> hq[4];
     if (q)
         for (int i = 0; i < 4; i++) hq[i].assign(N, 0.f);
@@ -388,23 +517,41 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
     deriv_table(nzz, j.kzscr, j.kzsci, &hmz[0]);         // v -> p, z: "sc" (acps4.cu:527)
     deriv_table(nzz, j.kzsmr, j.kzsmi, &hmz[nzz]);       // p -> v, z: "sm" (acps4.cu:595)
     lap_table(nzz, j.dz, &hmz[2 * nzz]);
-    // receivers sorted by column, with per-x-tile offsets
-    std::vector<int> order(j.nr);
-    for (int i = 0; i < j.nr; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return j.gxg[a] < j.gxg[c]; });
-    std::vector<int> hr(3 * std::max(j.nr, 1), 0), hoff(ntile_x + 1, 0);
+    // receivers sorted by column, with per-x-tile offsets (gather)
+    std::vector<int> gx(j.gxg, j.gxg + j.nr), order, hoff;
+    tile_csr(gx, cw, ntile_x, order, hoff);
+    std::vector<int> hr(3 * std::max(j.nr, 1), 0);
     for (int k = 0; k < j.nr; k++) {
         hr[k] = j.gxg[order[k]];
         hr[j.nr + k] = j.gzg[order[k]];
         hr[2 * j.nr + k] = order[k];
     }
-    for (int tl = 0, k = 0; tl <= ntile_x; tl++) {
-        while (k < j.nr && hr[k] < tl * cw) k++;
-        hoff[tl] = k;
+    // RTM data injection: one entry per distinct receiver cell, holding the
+    // last receiver mapped to it (rindex, acps4.cu:418-420), by row tile
+    // (acoustic) or column tile (viscoacoustic)
+    std::vector<int> hinj(3 * std::max(j.nr, 1), 0), hioff;
+    if (rtm) {
+        std::map<std::pair<int, int>, int> cell;
+        for (int i = 0; i < j.nr; i++) cell[{j.gxg[i], j.gzg[i]}] = i;
+        std::vector<int> cx, cz, ci, key;
+        for (auto &kv : cell) {
+            cx.push_back(kv.first.first);
+            cz.push_back(kv.first.second);
+            ci.push_back(kv.second);
+            key.push_back(q ? kv.first.first : kv.first.second);
+        }
+        std::vector<int> ord;
+        tile_csr(key, q ? cw : tz, ntile_inj, ord, hioff);
+        const int nc = (int)cx.size();
+        for (int k = 0; k < nc; k++) {
+            hinj[k] = cx[ord[k]];
+            hinj[j.nr + k] = cz[ord[k]];
+            hinj[2 * j.nr + k] = ci[ord[k]];
+        }
     }
     hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync((void *)d.dtbx, hbx.data(), 4 * N, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync((void *)d.dtbz, hbz.data(), 4 * N, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void *)base.dtbx, hbx.data(), 4 * N, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void *)base.dtbz, hbz.data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(f2, hf2.data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(f3, hf3.data(), 4 * N, hipMemcpyHostToDevice, s));
     if (q)
@@ -414,9 +561,15 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
     HIPCHK(hipMemcpyAsync(z1, h1z.data(), 4 * h1z.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(mx, hmx.data(), 8 * hmx.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(mz, hmz.data(), 8 * hmz.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync((void *)d.rik, j.rik, 4 * (size_t)j.nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void *)base.rik, j.rik, 4 * (size_t)j.nt, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(rcv, hr.data(), 4 * hr.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync((void *)d.rcv_off, hoff.data(), 4 * hoff.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void *)base.rcv_off, hoff.data(), 4 * hoff.size(), hipMemcpyHostToDevice, s));
+    if (rtm) {
+        if (j.nr > 0)
+            HIPCHK(hipMemcpyAsync((void *)base.obs, j.obs, 4 * (size_t)j.nt * j.nr, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync((void *)base.inj_x, hinj.data(), 4 * hinj.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync((void *)base.inj_off, hioff.data(), 4 * hioff.size(), hipMemcpyHostToDevice, s));
+    }
     // the host vectors above must outlive the copies
     HIPCHK(hipStreamSynchronize(s));
     return CQ_OK;
@@ -453,6 +606,13 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     }
     std::unique_ptr<cq_batch_s> b(new cq_batch_s());
     b->ctx = ctx;
+    b->rtm = is_rtm(jobs[0].kind);
+    for (int i = 1; i < njobs; i++) {
+        if (is_rtm(jobs[i].kind) != b->rtm)
+            return set_err(CQ_EINVAL, "a batch holds either modeling or RTM shots (job %d differs)", i);
+        if (b->rtm && (jobs[i].nt != jobs[0].nt || jobs[i].snaps != jobs[0].snaps))
+            return set_err(CQ_EINVAL, "RTM shots of one batch share nt and snaps (job %d differs)", i);
+    }
     // Tile shapes: the largest tiles whose lines fit one workgroup's per-stage
     // capacity and the LDS for every shot of the batch.
     int tz = 16, cw = 32, tk = 8;
@@ -464,8 +624,7 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
             return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nxx);
         if (!make_plan(j.nzz, pz))
             return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nzz);
-        const bool q = j.kind == CQ_KIND_VISCO;
-
+        const bool q = is_q(j.kind);
         while (tz > 8 && ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax)) tz /= 2;
         while (cw > 2 && ((cw / 2) * j.nzz > plan_capacity(pz) ||
                           (size_t)(1 + cw / 2) * j.nzz * 8 + (q ? (size_t)cw * j.nzz * 4 : 0) > kLdsMax))
@@ -473,8 +632,8 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
         if (q)
             while (tk > 2 && (tk * j.nxx > plan_capacity(px) || (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)) tk /= 2;
         if ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax ||
-            (cw / 2) * j.nzz > plan_capacity(pz) || (q && (tk * j.nxx > plan_capacity(px) ||
-                                                           (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)))
+            (cw / 2) * j.nzz > plan_capacity(pz) ||
+            (q && (tk * j.nxx > plan_capacity(px) || (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)))
             return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", i, j.nxx, j.nzz);
         maxnxx = std::max(maxnxx, j.nxx);
         maxnzz = std::max(maxnzz, j.nzz);
@@ -482,27 +641,31 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     b->tz = tz;
     b->cw = cw;
     b->tk = tk;
+    b->snaps = b->rtm ? jobs[0].snaps : 0;
     b->jobs.resize(njobs);
-    std::vector<int2> items[P_COUNT];
+    std::vector<int2> items[2][P_COUNT];
     for (int i = 0; i < njobs; i++) {
         int rc = build_job(b.get(), jobs[i], b->jobs[i]);
         if (rc) return rc;
         const JobH &h = b->jobs[i];
         b->tmax = std::max(b->tmax, h.nt);
-        if (h.kind == CQ_KIND_ACOUSTIC) {
-            for (int z0 = 0; z0 < h.nzz; z0 += tz) items[P_AROW].push_back(make_int2(i, z0));
-            for (int x0 = 0; x0 < h.nxx; x0 += cw) items[P_ACOL].push_back(make_int2(i, x0));
-        } else {
-            for (int x0 = 0; x0 < h.nxx; x0 += cw) {
-                items[P_QC1].push_back(make_int2(i, x0));
-                items[P_QC2].push_back(make_int2(i, x0));
-                items[P_QC3].push_back(make_int2(i, x0));
+        for (int r = 0; r < h.nroles; r++) {
+            std::vector<int2> *it = items[r];
+            if (!is_q(h.kind)) {
+                for (int z0 = 0; z0 < h.nzz; z0 += tz) it[P_AROW].push_back(make_int2(i, z0));
+                for (int x0 = 0; x0 < h.nxx; x0 += cw) it[P_ACOL].push_back(make_int2(i, x0));
+            } else {
+                for (int x0 = 0; x0 < h.nxx; x0 += cw) {
+                    it[P_QC1].push_back(make_int2(i, x0));
+                    it[P_QC2].push_back(make_int2(i, x0));
+                    it[P_QC3].push_back(make_int2(i, x0));
+                }
+                for (int k0 = 0; k0 < h.d[0].nzh; k0 += tk) {
+                    it[P_QR1].push_back(make_int2(i, k0));
+                    it[P_QR2].push_back(make_int2(i, k0));
+                }
+                for (int z0 = 0; z0 < h.nzz; z0 += tz) it[P_QR3].push_back(make_int2(i, z0));
             }
-            for (int k0 = 0; k0 < h.d.nzh; k0 += tk) {
-                items[P_QR1].push_back(make_int2(i, k0));
-                items[P_QR2].push_back(make_int2(i, k0));
-            }
-            for (int z0 = 0; z0 < h.nzz; z0 += tz) items[P_QR3].push_back(make_int2(i, z0));
         }
     }
     const size_t lrow = (size_t)(1 + tz / 2) * maxnxx * 8;
@@ -516,89 +679,102 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     b->lds[P_QR1] = (size_t)(1 + 2 * tk) * maxnxx * 8;
     b->lds[P_QR2] = (size_t)(1 + tk) * maxnxx * 8;
     for (int p = 0; p < P_COUNT; p++)
-        if (!items[p].empty() && b->lds[p] > kLdsMax)
+        if (!items[0][p].empty() && b->lds[p] > kLdsMax)
             return set_err(CQ_EUNSUPPORTED, "pass %s needs %zu B of LDS", kPassName[p], b->lds[p]);
-    std::vector<JobD> hd(njobs);
-    for (int i = 0; i < njobs; i++) hd[i] = b->jobs[i].d;
-    HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * njobs));
-    HIPCHK(hipMemcpy(b->djobs, hd.data(), sizeof(JobD) * njobs, hipMemcpyHostToDevice));
-    for (int p = 0; p < P_COUNT; p++) {
-        if (items[p].empty()) continue;
-        std::vector<int2> o = xcd_order(items[p]);
-        HIPCHK(hipMalloc(&b->ditems[p], sizeof(int2) * o.size()));
-        HIPCHK(hipMemcpy(b->ditems[p], o.data(), sizeof(int2) * o.size(), hipMemcpyHostToDevice));
-        b->nitems[p] = (int)o.size();
+    // job table: role 0 entries, then role 1 entries (same index + njobs)
+    std::vector<JobD> hd(2 * (size_t)njobs);
+    for (int i = 0; i < njobs; i++) {
+        hd[i] = b->jobs[i].d[0];
+        hd[njobs + i] = b->jobs[i].nroles > 1 ? b->jobs[i].d[1] : b->jobs[i].d[0];
     }
+    HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * hd.size()));
+    HIPCHK(hipMemcpy(b->djobs, hd.data(), sizeof(JobD) * hd.size(), hipMemcpyHostToDevice));
+    for (int r = 0; r < 2; r++)
+        for (int p = 0; p < P_COUNT; p++) {
+            if (items[r][p].empty()) continue;
+            std::vector<int2> o = xcd_order(items[r][p]);
+            HIPCHK(hipMalloc(&b->ditems[r][p], sizeof(int2) * o.size()));
+            HIPCHK(hipMemcpy(b->ditems[r][p], o.data(), sizeof(int2) * o.size(), hipMemcpyHostToDevice));
+            b->nitems[r][p] = (int)o.size();
+        }
     *out = b.release();
     return CQ_OK;
 }
 
 template <typename K>
-static int launch(cq_batch b, K kernel, int pass, int step)
+static int launch(cq_batch b, K kernel, int pass, int role, int step, int t_aux, int flags)
 {
     const size_t lds = b->lds[pass];
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kernel, dim3(b->nitems[pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs, b->ditems[pass],
-                       step);
+    const JobD *table = b->djobs + (role ? b->jobs.size() : 0);
+    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, b->ctx->stream, table,
+                       b->ditems[role][pass], step, t_aux, flags);
     HIPCHK(hipGetLastError());
     return CQ_OK;
 }
 
-#define CQ_DISPATCH_TZ(K, pass)                                                   \
-    (b->tz == 16 ? launch(b, K<16>, pass, step) : launch(b, K<8>, pass, step))
-#define CQ_DISPATCH_CW(K, pass)                                                   \
-    (b->cw == 32   ? launch(b, K<32>, pass, step)                                 \
-     : b->cw == 16 ? launch(b, K<16>, pass, step)                                 \
-     : b->cw == 8  ? launch(b, K<8>, pass, step)                                  \
-     : b->cw == 4  ? launch(b, K<4>, pass, step)                                  \
-                   : launch(b, K<2>, pass, step))
-#define CQ_DISPATCH_TK(K, pass)                                                   \
-    (b->tk == 8 ? launch(b, K<8>, pass, step)                                     \
-     : b->tk == 4 ? launch(b, K<4>, pass, step) : launch(b, K<2>, pass, step))
+#define CQ_ARGS pass, role, step, t_aux, flags
+#define CQ_DISPATCH_TZ(K)                                                          \
+    (b->tz == 16 ? launch(b, K<16>, CQ_ARGS) : launch(b, K<8>, CQ_ARGS))
+#define CQ_DISPATCH_CW(K)                                                          \
+    (b->cw == 32   ? launch(b, K<32>, CQ_ARGS)                                     \
+     : b->cw == 16 ? launch(b, K<16>, CQ_ARGS)                                     \
+     : b->cw == 8  ? launch(b, K<8>, CQ_ARGS)                                      \
+     : b->cw == 4  ? launch(b, K<4>, CQ_ARGS)                                      \
+                   : launch(b, K<2>, CQ_ARGS))
+#define CQ_DISPATCH_TK(K)                                                          \
+    (b->tk == 8 ? launch(b, K<8>, CQ_ARGS)                                         \
+     : b->tk == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
 
-static int launch_pass(cq_batch b, int pass, int step)
+static int launch_pass(cq_batch b, int pass, int role, int step, int t_aux, int flags)
 {
     switch (pass) {
-    case P_AROW: return CQ_DISPATCH_TZ(k_row_ac, pass);
-    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, pass);
-    case P_QC1: return CQ_DISPATCH_CW(k_q_c1, pass);
-    case P_QR1: return CQ_DISPATCH_TK(k_q_r1, pass);
-    case P_QC2: return CQ_DISPATCH_CW(k_q_c2, pass);
-    case P_QR2: return CQ_DISPATCH_TK(k_q_r2, pass);
-    case P_QC3: return CQ_DISPATCH_CW(k_q_c3, pass);
-    case P_QR3: return CQ_DISPATCH_TZ(k_q_r3, pass);
+    case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
+    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac);
+    case P_QC1: return CQ_DISPATCH_CW(k_q_c1);
+    case P_QR1: return CQ_DISPATCH_TK(k_q_r1);
+    case P_QC2: return CQ_DISPATCH_CW(k_q_c2);
+    case P_QR2: return CQ_DISPATCH_TK(k_q_r2);
+    case P_QC3: return CQ_DISPATCH_CW(k_q_c3);
+    case P_QR3: return CQ_DISPATCH_TZ(k_q_r3);
     }
     return set_err(CQ_EINVAL, "unknown pass %d", pass);
+}
+
+// One time step of every wavefield of the given role.
+static int one_step(cq_batch b, int role, int step, int t_aux, int flags)
+{
+    if (b->timing) {
+        hipEvent_t e0;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventRecord(e0, b->ctx->stream));
+        b->ev.push_back(e0);
+        b->ev_pass.push_back(-1);
+    }
+    for (int p = 0; p < P_COUNT; p++) {
+        if (b->nitems[role][p] == 0) continue;
+        int rc = launch_pass(b, p, role, step, t_aux, flags);
+        if (rc) return rc;
+        if (b->timing) {
+            hipEvent_t e1;
+            HIPCHK(hipEventCreate(&e1));
+            HIPCHK(hipEventRecord(e1, b->ctx->stream));
+            b->ev.push_back(e1);
+            b->ev_pass.push_back(p);
+        }
+    }
+    return CQ_OK;
 }
 
 extern "C" int cq_batch_step(cq_batch b, int nsteps)
 {
     if (!b || nsteps < 0) return set_err(CQ_EINVAL, "cq_batch_step: bad arguments");
+    if (b->rtm) return set_err(CQ_ESTATE, "cq_batch_step: RTM batches run with cq_batch_run");
     HIPCHK(hipSetDevice(b->ctx->device));
     for (int s = 0; s < nsteps && b->t < b->tmax; s++) {
-        int nev = 0;
-        if (b->timing) {
-            hipEvent_t e0;
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventRecord(e0, b->ctx->stream));
-            b->ev.push_back(e0);
-            b->ev_pass.push_back(-1);
-        }
-        for (int p = 0; p < P_COUNT; p++) {
-            if (b->nitems[p] == 0) continue;
-            int rc = launch_pass(b, p, b->t);
-            if (rc) return rc;
-            if (b->timing) {
-                hipEvent_t e1;
-                HIPCHK(hipEventCreate(&e1));
-                HIPCHK(hipEventRecord(e1, b->ctx->stream));
-                b->ev.push_back(e1);
-                b->ev_pass.push_back(p);
-                nev++;
-            }
-        }
-        if (b->timing) b->step_nev.push_back(nev);
+        int rc = one_step(b, 0, b->t, b->t - 1, 0);
+        if (rc) return rc;
         b->t++;
     }
     return CQ_OK;
@@ -611,12 +787,88 @@ extern "C" int cq_batch_sync(cq_batch b)
     return CQ_OK;
 }
 
+__global__ void k_add_point(float *p, float v) { *p += v; }
+
+// Checkpoint copies for every shot: dir 1 state -> slot, 0 slot -> state
+// (storecheckpoints, sgpstd-qrtm/acps4.cu:1797-1844).  Slot 0 also gets the
+// reference's wavelet sample rik[0] dt 1000 added at the source
+// (addsource, :1846-1863) when taken at capo 0.
+static int checkpoint(cq_batch b, int dir, int slot, bool add_src0)
+{
+    hipStream_t s = b->ctx->stream;
+    for (auto &h : b->jobs) {
+        char *ck = h.ckpt + (size_t)slot * h.state_bytes;
+        if (dir) {
+            HIPCHK(hipMemcpyAsync(ck, h.state[0], h.state_bytes, hipMemcpyDeviceToDevice, s));
+            if (add_src0) {
+                const size_t off = (size_t)((char *)h.d[0].p - h.state[0]);
+                hipLaunchKernelGGL(k_add_point, dim3(1), dim3(1), 0, s,
+                                   reinterpret_cast<float *>(ck + off) + h.src_cell, h.src0);
+                HIPCHK(hipGetLastError());
+            }
+        } else {
+            HIPCHK(hipMemcpyAsync(h.state[0], ck, h.state_bytes, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    return CQ_OK;
+}
+
+// The revolve-driven RTM loop (sgpstd-qrtm/acps4.cu:447-905,
+// sgpstd-qrtm/acsglw.cu:470-1040) for every shot of the batch in lock step.
+static int rtm_run(cq_batch b)
+{
+    if (b->rtm_done) return CQ_OK;
+    cqh_revolve rv;
+    cqh_revolve_init(&rv, b->tmax, b->snaps);
+    int ts = 0, restart = 0, rc = CQ_OK;
+    for (;;) {
+        const int a = cqh_revolve_next(&rv);
+        if (a == CQH_RV_TAKESHOT) {
+            rc = checkpoint(b, 1, rv.check, rv.capo == 0);
+        } else if (a == CQH_RV_ADVANCE) {
+            for (int t = ts; t <= rv.capo && rc == CQ_OK; t++)
+                rc = one_step(b, 0, t, t - 1, restart ? 0 : CQ_F_SSG);
+            ts = rv.capo + 1;
+        } else if (a == CQH_RV_FIRSTURN) {
+            restart = 1;
+            rc = one_step(b, 1, b->tmax - 1, b->tmax, 0);
+        } else if (a == CQH_RV_YOUTURN) {
+            rc = one_step(b, 1, rv.capo, rv.capo + 1, 0);
+        } else if (a == CQH_RV_RESTORE) {
+            rc = checkpoint(b, 0, rv.check, false);
+            ts = rv.capo + 1;
+        } else if (a == CQH_RV_TERMINATE) {
+            break;
+        } else {
+            b->rv_info = rv.info;
+            return set_err(CQ_EINVAL, "revolve error, info %d", rv.info);
+        }
+        if (rc) return rc;
+    }
+    b->rtm_done = 1;
+    b->t = b->tmax;
+    return CQ_OK;
+}
+
 extern "C" int cq_batch_run(cq_batch b)
 {
     if (!b) return set_err(CQ_EINVAL, "cq_batch_run: null batch");
-    int rc = cq_batch_step(b, b->tmax - b->t);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    int rc = b->rtm ? rtm_run(b) : cq_batch_step(b, b->tmax - b->t);
     if (rc) return rc;
     return cq_batch_sync(b);
+}
+
+extern "C" int cq_batch_read_image(cq_batch b, int job, float *img, float *ssg)
+{
+    if (!b || job < 0 || job >= (int)b->jobs.size() || !img || !ssg)
+        return set_err(CQ_EINVAL, "cq_batch_read_image: bad arguments");
+    if (!b->rtm) return set_err(CQ_ESTATE, "cq_batch_read_image: not an RTM batch");
+    const JobH &h = b->jobs[job];
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    HIPCHK(hipMemcpy(img, h.d[0].img, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ssg, h.d[0].ssg, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
+    return CQ_OK;
 }
 
 extern "C" int cq_batch_read_rec(cq_batch b, int job, float *rec)
@@ -624,7 +876,8 @@ extern "C" int cq_batch_read_rec(cq_batch b, int job, float *rec)
     if (!b || job < 0 || job >= (int)b->jobs.size() || !rec) return set_err(CQ_EINVAL, "cq_batch_read_rec: bad arguments");
     const JobH &h = b->jobs[job];
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    if (h.nr > 0) HIPCHK(hipMemcpy(rec, h.d.rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
+    if (b->rtm) return set_err(CQ_ESTATE, "cq_batch_read_rec: RTM batches have no gathers");
+    if (h.nr > 0) HIPCHK(hipMemcpy(rec, h.d[0].rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
     return CQ_OK;
 }
 
@@ -633,7 +886,7 @@ extern "C" int cq_batch_read_p(cq_batch b, int job, float *p)
     if (!b || job < 0 || job >= (int)b->jobs.size() || !p) return set_err(CQ_EINVAL, "cq_batch_read_p: bad arguments");
     const JobH &h = b->jobs[job];
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    HIPCHK(hipMemcpy2D(p, sizeof(float) * h.nzz, h.d.p, sizeof(float) * h.ldz, sizeof(float) * h.nzz, h.nxx,
+    HIPCHK(hipMemcpy2D(p, sizeof(float) * h.nzz, h.d[0].p, sizeof(float) * h.ldz, sizeof(float) * h.nzz, h.nxx,
                        hipMemcpyDeviceToHost));
     return CQ_OK;
 }
@@ -664,7 +917,6 @@ extern "C" int cq_batch_enable_timing(cq_batch b, int on)
     for (auto e : b->ev) (void)hipEventDestroy(e);
     b->ev.clear();
     b->ev_pass.clear();
-    b->step_nev.clear();
     b->timing = on != 0;
     return CQ_OK;
 }
@@ -750,7 +1002,7 @@ extern "C" int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec)
     for (int t = 0; t < h.nt && rc == CQ_OK; t++) {
         rc = cq_batch_step(b, 1);
         if (rc == CQ_OK && h.nr > 0) {
-            hipError_t e = hipMemcpyAsync(pinned, h.d.rec + (size_t)t * h.nr, sizeof(float) * h.nr,
+            hipError_t e = hipMemcpyAsync(pinned, h.d[0].rec + (size_t)t * h.nr, sizeof(float) * h.nr,
                                           hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) rc = set_err(CQ_EDEVICE, "plain gather copy: %s", hipGetErrorString(e));

@@ -81,8 +81,9 @@ __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
 }
 
 // dvx/dx (in LDS) with its x-CPML memory variable -> dvx for the next step
-// (update_vxx_vzz, acps4.cu:1111-1144)
-template <int Q>
+// (update_vxx_vzz, acps4.cu:1111-1144).  TO_DIV: add dvz (already computed
+// by the column pass) and store div instead (sumpxpz, acps4.cu:1146-1172).
+template <int Q, bool TO_DIV = false>
 __device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, int z0)
 {
     const int nxx = J.nxx, ldz = J.ldz, pml = J.pml;
@@ -103,7 +104,13 @@ __device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, 
             st4(psp, ps);
             dv.x += ps.x; dv.y += ps.y; dv.z += ps.z; dv.w += ps.w;
         }
-        st4(J.dvx + g, dv);
+        if (TO_DIV) {
+            const float4 dz = ld4(J.dvz + g);
+            dv.x += dz.x; dv.y += dz.y; dv.z += dz.z; dv.w += dz.w;
+            st4(J.div + g, dv);
+        } else {
+            st4(J.dvx + g, dv);
+        }
     }
 }
 
@@ -174,8 +181,9 @@ __device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
 }
 
 // dvz/dz with z-CPML, div = dvx + dvz (update_vxx_vzz + sumpxpz,
-// acps4.cu:1111-1172); stores div and leaves it in LDS
-template <int CW>
+// acps4.cu:1111-1172); stores div and leaves it in LDS.  TO_DVZ: store dvz
+// alone (the row pass adds dvx).
+template <int CW, bool TO_DVZ = false>
 __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
@@ -189,7 +197,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
             continue;
         }
         const size_t g = (size_t)ix * J.ldz + 4 * zq;
-        const float4 dvx = ld4(J.dvx + g);
+        const float4 dvx = TO_DVZ ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(J.dvx + g);
         float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -207,7 +215,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
             f4set(dv, k, s);
             line[2 * iz] = s;
         }
-        st4(J.div + g, dv);
+        st4((TO_DVZ ? J.dvz : J.div) + g, dv);
     }
 }
 
@@ -308,9 +316,13 @@ __device__ __forceinline__ float frac_symbol(int which, float k2)
 
 // buf[l][kx] = (save ? copy : buf)[l][kx] * symbol(kx, k0 + l) / N; with
 // save != null the unmultiplied values are first copied there.
+// which = 4: the stabilised compensation multiplier of Q-RTM,
+//   S = k - 2 e k (1 + tc dt k) / (exp(-tc tt k) + e exp(tc dt k)),  k = |k|^1.01,
+// e = eta * 1e-6, tc = max tao, tt = t_aux dt (multt_kernel,
+// sgpstd-qrtm/acsglw.cu:1895-1924).
 template <int TK>
 __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *save, const float2 *from, int k0,
-                                          int which)
+                                          int which, float tt = 0.f)
 {
     const int nxx = J.nxx;
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
@@ -319,7 +331,16 @@ __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *sa
         const float fkx = (float)ikx * J.dkx;
         const float fkz = (float)(k0 + l) * J.dkz;
         const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
-        const float m = frac_symbol(which, k2) * J.invN;
+        float m;
+        if (which == 4) {
+            const float k = frac_symbol(3, k2);
+            const float es = J.eta * 1.0e-6f;
+            const float a = J.taoc * J.dt * k;
+            m = k - 2.0f * es * k * (1.0f + a) / (expf(-J.taoc * tt * k) + es * expf(a));
+        } else {
+            m = frac_symbol(which, k2);
+        }
+        m *= J.invN;
         float2 v = from ? from[e] : buf[e];
         if (save) save[e] = v;
         buf[e] = make_float2(v.x * m, v.y * m);
@@ -329,17 +350,36 @@ __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *sa
 // ===========================================================================
 // acoustic passes (sgpstd-qmodeling/acps4.cu:508-645)
 // ===========================================================================
+// Kernel arguments: step = time index of this step (wavelet sample for the
+// source wavefield, data sample for the receiver wavefield); t_aux = time
+// index the stabilised aux multiplier uses; flags = CQ_F_* bits.
 #define CQ_ITEM_PROLOGUE                      \
     const int2 it = items[blockIdx.x];        \
     if (it.x < 0) return;                     \
     const JobD &J = jobs[it.x];               \
     const int t = step;                       \
-    if (t >= J.nt) return;
+    if (t >= J.nt) return;                    \
+    (void)t_aux;                              \
+    (void)flags;
+
+// dst[interior(ix, iz0 + k)] += a.k * b.k for the 4 depths of a float4
+// (image: rtm_image, sgpstd-qrtm/acps4.cu:1694-1721; illumination: addsa_kernel,
+// :1656)
+__device__ __forceinline__ void interior_accum(const JobD &J, float *dst, int ix, int iz0, float4 a, float4 b)
+{
+    const int xi = ix - J.pml;
+    if (xi < 0 || xi >= J.nx) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int zi = iz0 + k - J.pml;
+        if (zi >= 0 && zi < J.nz) dst[(size_t)xi * J.nz + zi] += f4get(a, k) * f4get(b, k);
+    }
+}
 
 // row pass: L = Lx + Lz, p update + source, dp/dx -> vx, dvx/dx -> dvx
 template <int TZ>
 __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                  int step)
+                                                  int step, int t_aux, int flags)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
@@ -352,8 +392,11 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     cq_fft_fwd(J.px, buf, nxx, L, tw, J.mx2);
     cq_fft_inv(J.px, buf, nxx, L, tw);
 
-    // p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498)
-    const float src = J.rik[t] * J.src_scale;
+    // p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498);
+    // illumination ssg += p^2 during the first forward sweep of RTM
+    // (sgpstd-qrtm/acps4.cu:1623-1659)
+    const bool rcv = J.role == 1;
+    const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int ix = e / Q, q = e % Q;
         const size_t g = (size_t)ix * ldz + z0 + 4 * q;
@@ -365,15 +408,38 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
         pn.y = p.y + c1.y * d.y - c2.y * (a.y + lz.y);
         pn.z = p.z + c1.z * d.z - c2.z * (b.x + lz.z);
         pn.w = p.w + c1.w * d.w - c2.w * (b.y + lz.w);
-        if (ix == J.xsg) {
-            const int dz = J.zsg - (z0 + 4 * q);
-            if (dz >= 0 && dz < 4) f4set(pn, dz, f4get(pn, dz) + src);
+        if (!rcv) {
+            if (ix == J.xsg) {
+                const int dz = J.zsg - (z0 + 4 * q);
+                if (dz >= 0 && dz < 4) f4set(pn, dz, f4get(pn, dz) + src);
+            }
+            st4(J.p + g, pn);
+            if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, z0 + 4 * q, pn, pn);
         }
-        st4(J.p + g, pn);
         buf[(2 * q) * nxx + ix] = make_float2(pn.x, pn.y);
         buf[(2 * q + 1) * nxx + ix] = make_float2(pn.z, pn.w);
     }
     __syncthreads();
+    if (rcv) {
+        // receiver wavefield: p = obs at the receiver cells (addsa_b_kernel,
+        // sgpstd-qrtm/acps4.cu:1662-1692), then img += p_src p (rtm_image)
+        const int tile = z0 / TZ;
+        for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT) {
+            const int dz = J.inj_z[r] - z0;
+            float *f = reinterpret_cast<float *>(buf + (dz >> 1) * nxx + J.inj_x[r]) + (dz & 1);
+            *f = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
+            const int ix = e / Q, q = e % Q;
+            const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+            const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+            const float4 pn = make_float4(a.x, a.y, b.x, b.y);
+            st4(J.p + g, pn);
+            interior_accum(J, J.img, ix, z0 + 4 * q, pn, ld4(J.psrc + g));
+        }
+        __syncthreads();
+    }
     cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
     cq_fft_inv(J.px, buf, nxx, L, tw);
     row_vx_phase<Q>(J, buf, z0);
@@ -386,7 +452,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
 // column pass: gather, dp/dz -> vz, dvz/dz -> div, Lz = d2/dz2 div
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                  int step)
+                                                  int step, int t_aux, int flags)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -398,7 +464,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
     col_fill<CW>(bufs, J, J.p, x0);
     __syncthreads();
     // receiver gather of the updated p (acps4.cu:611-621)
-    col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
     cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzp);
     cq_fft_inv(J.pz, buf, nzz, L, tw);
     col_vz_phase<CW>(J, bufs, x0);
@@ -423,17 +489,20 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
 
 // ===========================================================================
 // viscoacoustic passes (sgpstd-qmodeling/acsglw.cu:511-683)
-// One step = C1 R1 C2 R2 C3 R3:
-//   C1 (cols) : dp/dz -> vz, dvz/dz -> div; half spectra F_z p (s2), F_z div (s1)
+// One step = C1 R1 C2 R2 C3 R3; between steps the state is the reference's
+// state after the step (p, vx, vz, CPML) plus quantities derived from it:
+// div and the dvx/dvz CPML memory of the next step, and F_z p (s2).
+//   C1 (cols) : F_z div -> s1
 //   R1 (kz)   : F_x, x3 symbols -> A (s1), K (s3), B (s4); F_x p x |k|^1.01 -> aux (s2)
 //   C2 (cols) : F_z^-1 A, K, B; total (sum_total, acsglw.cu:1213-1245); F_z total (s1)
 //   R2 (kz)   : F_x total x |k|^1.01 -> T (s1)
-//   C3 (cols) : F_z^-1 T, aux; p update + source (acsglw.cu:1500-1537); gather
-//   R3 (rows) : dp/dx -> vx, dvx/dx -> dvx
+//   C3 (cols) : F_z^-1 T, aux; p update + source (acsglw.cu:1500-1537); gather;
+//               F_z p -> s2; dp/dz -> vz; dvz/dz -> dvz
+//   R3 (rows) : dp/dx -> vx; dvx/dx; div = dvx + dvz
 // ===========================================================================
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -442,17 +511,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
     load_tw(tw, J.twz, nzz);
-    col_fill<CW>(bufs, J, J.p, x0);
-    __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
-    col_extract<CW>(J, buf, x0, J.s2, J.mzp);
-    __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
-    col_vz_phase<CW>(J, bufs, x0);
-    __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
-    col_div_phase<CW>(J, bufs, x0);
+    col_fill<CW>(bufs, J, J.div, x0);
     __syncthreads();
     cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
@@ -460,7 +519,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
 
 template <int TK>
 __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y;
@@ -483,7 +542,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
     krow_fill<TK>(buf, J, J.s2, k0);
     __syncthreads();
     cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
-    krow_mult<TK>(J, buf, nullptr, nullptr, k0, 3);
+    krow_mult<TK>(J, buf, nullptr, nullptr, k0, J.stab ? 4 : 3, (float)t_aux * J.dt);
     __syncthreads();
     cq_fft_inv(J.px, buf, nxx, TK, tw);
     krow_store<TK>(buf, J, J.s2, k0);
@@ -491,7 +550,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
 
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -568,7 +627,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 
 template <int TK>
 __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y;
@@ -586,7 +645,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, c
 
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -612,8 +671,12 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
     __syncthreads();
     cq_fft_inv(J.pz, buf, nzz, L, tw);
-    // p += rho v^2 dt lam (total - dt tao T / 2) - dt tao aux; source
-    const float src = J.rik[t] * J.src_scale;
+    // p += rho v^2 dt lam (total - s dt tao T / 2) - s dt tao aux; source.
+    // s = +1 attenuates (acsglw.cu:1500-1537); s = -1 compensates (Q-RTM,
+    // sgpstd-qrtm/acsglw.cu:1769-1841)
+    const bool rcv = J.role == 1;
+    const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
+    const float s = J.qsign;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
@@ -621,26 +684,58 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         const size_t g = (size_t)ix * ldz + 4 * zq;
         const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
                      qe = ld4(J.qe + g);
-        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - f4get(qb, k) * acc[c * nzz + iz]) -
-                      f4get(qe, k) * line[2 * iz];
-            if (ix == J.xsg && iz == J.zsg) v += src;
-            f4set(pn, k, v);
+            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * acc[c * nzz + iz]) -
+                      s * f4get(qe, k) * line[2 * iz];
+            if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
             acc[c * nzz + iz] = v;
         }
-        st4(J.p + g, pn);
     }
     __syncthreads();
-    col_gather<CW>(J, x0, t, [&](int c, int z) { return acc[c * nzz + z]; });
+    if (rcv) {
+        // receiver wavefield: p = obs at the receiver cells (addsa_b_kernel,
+        // sgpstd-qrtm/acsglw.cu:1809-1841)
+        const int tile = x0 / CW;
+        for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT)
+            acc[(J.inj_x[r] - x0) * nzz + J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
+        const size_t g = (size_t)ix * ldz + 4 * zq;
+        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (4 * zq + k < nzz) f4set(pn, k, acc[c * nzz + 4 * zq + k]);
+        st4(J.p + g, pn);
+        if (rcv) interior_accum(J, J.img, ix, 4 * zq, pn, ld4(J.psrc + g));
+        else if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, 4 * zq, pn, pn);
+    }
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return acc[c * nzz + z]; });
+    // p -> LDS (packed columns) for F_z p and dp/dz
+    for (int e = threadIdx.x; e < CW * nzz; e += CQ_NT) {
+        const int c = e / nzz, iz = e - c * nzz;
+        col_line(bufs, nzz, c)[2 * iz] = (x0 + c < J.nxx) ? acc[c * nzz + iz] : 0.f;
+    }
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
+    __syncthreads();
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_vz_phase<CW>(J, bufs, x0);
+    __syncthreads();
+    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
+    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    col_div_phase<CW, true>(J, bufs, x0);
 }
 
 template <int TZ>
 __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step)
+                                                int step, int t_aux, int flags)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
@@ -656,5 +751,5 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, c
     __syncthreads();
     cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
     cq_fft_inv(J.px, buf, nxx, L, tw);
-    row_dvx_phase<Q>(J, buf, z0);
+    row_dvx_phase<Q, true>(J, buf, z0);
 }

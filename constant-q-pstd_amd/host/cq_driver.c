@@ -1,0 +1,244 @@
+/*
+ * cq_driver.c -- the aps / vaps / artm / qrtm drivers on the MI355X engine.
+ *
+ * Replaces the reference's MPI drivers sgpstd-qmodeling/acousticsgps4.c (aps),
+ * sgpstd-qmodeling/visacousticsgps4.c (vaps), sgpstd-qrtm/main.c (artm) and
+ * sgpstd-qrtm/acrtmain.c (qrtm): the same parameter files, geometry and model
+ * files and the same outputs (reca<shot>_<nr>.bin / recva<shot>_<nr>.bin gathers,
+ * RTM.BIN / QRTM.BIN = [image | illumination]).  The MPI shot loop becomes
+ * on-GPU batching: every shot of the run advances in the same launches
+ * (modeling: chunks of `batch` shots; RTM: all shots in lock step).
+ *
+ * usage: <aps|vaps|artm|qrtm> [parameter file] [gather directory] [batch]
+ *   parameter file   default ../input/<reference name>.txt
+ *   gather directory default ../obsdata (modeling output, as acps4.cu:694)
+ *   batch            shots per modeling batch, default 64
+ * The mode is taken from the program name (argv[0]) or CQ_MODE.
+ */
+#include <libgen.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "cq.h"
+#include "cq_host.h"
+
+#define DIE(...)                                   \
+    do {                                           \
+        fprintf(stderr, __VA_ARGS__);              \
+        fputc('\n', stderr);                       \
+        return 1;                                  \
+    } while (0)
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+typedef struct {
+    float *vp, *rho, *bbx, *bbz, *gam, *lam, *vwd, *tao;
+    int *gxg, *gzg;
+    float *obs;
+    cqh_shot_geom geo;
+    int shotid, nr;
+} shot_buf;
+
+static float *win_field(const float *full, int nxt, int nzt, int nx, int nz, int pml, float mlx, float dx)
+{
+    const int nxx = nx + 2 * pml, nzz = nz + 2 * pml;
+    float *w = (float *)calloc((size_t)nxx * nzz, sizeof(float));
+    cqh_select_part_model(pml, mlx, dx, nxt, nzt, nx, nz, nxx, nzz, full, w);
+    cqh_expand_pml(nx, nz, nxx, nzz, pml, w);
+    return w;
+}
+
+static void free_shot(shot_buf *s)
+{
+    free(s->vp); free(s->rho); free(s->bbx); free(s->bbz);
+    free(s->gam); free(s->lam); free(s->vwd); free(s->tao);
+    free(s->gxg); free(s->gzg); free(s->obs);
+    memset(s, 0, sizeof(*s));
+}
+
+int main(int argc, char **argv)
+{
+    const char *mode = getenv("CQ_MODE");
+    if (!mode) mode = basename(argv[0]);
+    int fmt;
+    const char *defpar;
+    if (!strcmp(mode, "aps")) { fmt = CQH_FMT_APS; defpar = "../input/ac2drealmodeling.txt"; }
+    else if (!strcmp(mode, "vaps")) { fmt = CQH_FMT_VAPS; defpar = "../input/visac2drealmodeling.txt"; }
+    else if (!strcmp(mode, "artm")) { fmt = CQH_FMT_ARTM; defpar = "../input/ac2drtm.txt"; }
+    else if (!strcmp(mode, "qrtm")) { fmt = CQH_FMT_QRTM; defpar = "../input/visac2drtm.txt"; }
+    else DIE("unknown mode %s (aps, vaps, artm, qrtm)", mode);
+    const int q = (fmt == CQH_FMT_VAPS || fmt == CQH_FMT_QRTM);
+    const int rtm = (fmt == CQH_FMT_ARTM || fmt == CQH_FMT_QRTM);
+    const char *parfile = argc > 1 ? argv[1] : defpar;
+    const char *outdir = argc > 2 ? argv[2] : "../obsdata";
+    const int batch = argc > 3 ? atoi(argv[3]) : 64;
+
+    cqh_params P;
+    if (cqh_read_params(parfile, fmt, &P)) DIE("cannot read parameter file %s", parfile);
+    const int shotnum = P.shotnum, tmax = P.tmax, nxt = P.nx, nzt = P.nz, pml = P.pml;
+    const float dt = P.dt_ms / 1000.0;     /* acousticsgps4.c:182 */
+    const float dx = P.dx, dz = P.dz, fd = P.fd;
+    const size_t nzxt = (size_t)nxt * nzt;
+
+    int *tracenum = (int *)malloc(sizeof(int) * shotnum);
+    float *shotxp = (float *)malloc(sizeof(float) * shotnum), *shotzp = (float *)malloc(sizeof(float) * shotnum);
+    float *vpt = (float *)malloc(sizeof(float) * nzxt), *rhot = (float *)malloc(sizeof(float) * nzxt);
+    float *bbxt = (float *)malloc(sizeof(float) * nzxt), *bbzt = (float *)malloc(sizeof(float) * nzxt);
+    float *qt = NULL, *gamt = NULL, *lamt = NULL, *vwdt = NULL, *taot = NULL;
+    if (cqh_read_i32(P.fname1, tracenum, shotnum)) DIE("Cannot open the tracenumber file %s", P.fname1);
+    if (cqh_read_f32(P.fname2, shotxp, shotnum)) DIE("Cannot open the shot x position file %s", P.fname2);
+    if (cqh_read_f32(P.fname3, shotzp, shotnum)) DIE("Cannot open the shot z position file %s", P.fname3);
+    if (cqh_read_f32(P.vpname, vpt, nzxt)) DIE("Cannot open the velocity file %s", P.vpname);
+    if (cqh_read_f32(P.rhoname, rhot, nzxt)) DIE("Cannot open the density file %s", P.rhoname);
+    float vmax, vmin;
+    cqh_get_max_min((int)nzxt, vpt, &vmax, &vmin);
+    printf(" CFL number= %f %f %f\n", cqh_check_stability(vmax, dt, dx, dz), vmax, vmin);
+    if (q) {
+        qt = (float *)malloc(sizeof(float) * nzxt);
+        gamt = (float *)malloc(sizeof(float) * nzxt); lamt = (float *)malloc(sizeof(float) * nzxt);
+        vwdt = (float *)malloc(sizeof(float) * nzxt); taot = (float *)malloc(sizeof(float) * nzxt);
+        if (cqh_read_f32(P.qname, qt, nzxt)) DIE("Cannot open the Q file %s", P.qname);
+        cqh_transfermodel((int)nzxt, fd, P.fo, vpt, qt, gamt, lamt, vwdt, taot);
+    }
+    cqh_bbx_bbz(nxt, nzt, rhot, bbxt, bbzt);
+    float *rik = (float *)malloc(sizeof(float) * tmax);
+    cqh_ricker(tmax, dt, fd, rik, P.flagsource);
+
+    const float xmax = (nxt - 1) * dx, zmax = (nzt - 1) * dz;
+    for (int s = 0; s < shotnum; s++)
+        if (shotxp[s] < 0 || shotxp[s] > xmax || shotzp[s] < 0 || shotzp[s] > zmax)
+            DIE("%d shot exceeds model!", s);
+    int nx = 0;
+    const float sroffmx = cqh_window_sroffmx(0, shotnum, tracenum, shotxp, P.fname4, &nx, P.disx, dx);
+    if (sroffmx < 0) DIE("Cannot open receiver positions file %s<shot>.bin", P.fname4);
+    const int nz = nzt, nxx = nx + 2 * pml, nzz = nz + 2 * pml;
+    printf(" window %d x %d (padded %d x %d), %d shots, tmax %d\n", nx, nz, nxx, nzz, shotnum, tmax);
+
+    float *iax = (float *)malloc(4 * nxx), *ibx = (float *)malloc(4 * nxx), *hax = (float *)malloc(4 * nxx),
+          *hbx = (float *)malloc(4 * nxx), *iaz = (float *)malloc(4 * nzz), *ibz = (float *)malloc(4 * nzz),
+          *haz = (float *)malloc(4 * nzz), *hbz = (float *)malloc(4 * nzz);
+    cqh_pml_coeffs(pml, fd, iax, ibx, iaz, ibz, hax, hbx, haz, hbz, nxx, nzz, dx, dz, dt);
+    float *kxsmr = (float *)malloc(4 * nxx), *kxscr = (float *)malloc(4 * nxx), *kxsmi = (float *)malloc(4 * nxx),
+          *kxsci = (float *)malloc(4 * nxx), *kzsmr = (float *)malloc(4 * nzz), *kzscr = (float *)malloc(4 * nzz),
+          *kzsmi = (float *)malloc(4 * nzz), *kzsci = (float *)malloc(4 * nzz);
+    cqh_shift_coeffs(kxsmr, kxscr, kzsmr, kzscr, kxsmi, kxsci, kzsmi, kzsci, dx, dz, nxx, nzz);
+
+    cq_ctx ctx;
+    if (cq_open(0, &ctx)) DIE("cq_open: %s", cq_last_error());
+    int nrmx = 0;
+    for (int s = 0; s < shotnum; s++) if (tracenum[s] > nrmx) nrmx = tracenum[s];
+    float *gxp = (float *)malloc(sizeof(float) * nrmx), *gzp = (float *)malloc(sizeof(float) * nrmx);
+    float *img_sp = NULL, *ssg_sp = NULL;
+    if (rtm) {
+        img_sp = (float *)calloc(nzxt, sizeof(float));
+        ssg_sp = (float *)calloc(nzxt, sizeof(float));
+    }
+    const double t0 = now_s();
+    const int chunk = rtm ? shotnum : (batch > 0 ? batch : 64);
+    for (int s0 = 0; s0 < shotnum; s0 += chunk) {
+        const int ns = (s0 + chunk <= shotnum) ? chunk : shotnum - s0;
+        shot_buf *sb = (shot_buf *)calloc(ns, sizeof(shot_buf));
+        cq_job_desc *jd = (cq_job_desc *)calloc(ns, sizeof(cq_job_desc));
+        for (int k = 0; k < ns; k++) {
+            const int sid = s0 + k;
+            shot_buf *b = &sb[k];
+            b->shotid = sid;
+            b->nr = tracenum[sid];
+            const float xs = shotxp[sid], zs = shotzp[sid];
+            const float mlx = xs - sroffmx - P.disx;
+            b->vp = win_field(vpt, nxt, nzt, nx, nz, pml, mlx, dx);
+            b->rho = win_field(rhot, nxt, nzt, nx, nz, pml, mlx, dx);
+            b->bbx = win_field(bbxt, nxt, nzt, nx, nz, pml, mlx, dx);
+            b->bbz = win_field(bbzt, nxt, nzt, nx, nz, pml, mlx, dx);
+            if (q) {
+                b->gam = win_field(gamt, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->lam = win_field(lamt, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->vwd = win_field(vwdt, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->tao = win_field(taot, nxt, nzt, nx, nz, pml, mlx, dx);
+            }
+            if (cqh_read_geophones(P.fname4, sid, b->nr, gxp) || cqh_read_geophones(P.fname5, sid, b->nr, gzp))
+                DIE("Cannot open receiver positions file for shot %d", sid);
+            b->gxg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
+            b->gzg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
+            if (cqh_shot_indices(pml, nxx, nzz, dx, dz, sroffmx, P.disx, xs, zs, b->nr, gxp, gzp, b->gxg, b->gzg,
+                                 &b->geo))
+                DIE("%d shot grid index exceeds model!", sid);
+            if (b->geo.nclamped) printf("warning: %d receivers of shot %d clamped out of the x pml\n", b->geo.nclamped, sid);
+            if (rtm) {
+                char name[512];
+                b->obs = (float *)malloc(sizeof(float) * (size_t)tmax * b->nr);
+                snprintf(name, sizeof name, "%s%d_%d.bin", P.recname, sid, b->nr);
+                if (cqh_read_f32(name, b->obs, (long)tmax * b->nr)) DIE("Cannot open observed data %s", name);
+                cqh_removedirect(tmax, b->nr, dt, fd, xs, zs, gxp, gzp, P.v0, b->obs);
+            }
+            cq_job_desc *j = &jd[k];
+            j->kind = rtm ? (q ? CQ_KIND_QRTM : CQ_KIND_RTM) : (q ? CQ_KIND_VISCO : CQ_KIND_ACOUSTIC);
+            j->nxx = nxx; j->nzz = nzz; j->pml = pml; j->nt = tmax; j->dt = dt;
+            j->vp = b->vp; j->rho = b->rho; j->bbx = b->bbx; j->bbz = b->bbz;
+            j->gam = b->gam; j->lam = b->lam; j->vwd = b->vwd; j->tao = b->tao;
+            j->iax = iax; j->ibx = ibx; j->hax = hax; j->hbx = hbx;
+            j->iaz = iaz; j->ibz = ibz; j->haz = haz; j->hbz = hbz;
+            j->kxsmr = kxsmr; j->kxsmi = kxsmi; j->kxscr = kxscr; j->kxsci = kxsci;
+            j->kzsmr = kzsmr; j->kzsmi = kzsmi; j->kzscr = kzscr; j->kzsci = kzsci;
+            j->dx = dx; j->dz = dz; j->rik = rik;
+            j->xsg = b->geo.xsg; j->zsg = b->geo.zsg;
+            j->nr = b->nr; j->gxg = b->gxg; j->gzg = b->gzg;
+            j->obs = b->obs; j->snaps = P.snapnum; j->etas = P.etas; j->etar = P.etar;
+            printf("shotid=%d nxx=%d nzz=%d xs=%f zs=%f\n", sid, nxx, nzz, xs, zs);
+        }
+        cq_batch bt;
+        if (cq_batch_create(ctx, jd, ns, &bt)) DIE("cq_batch_create: %s", cq_last_error());
+        if (cq_batch_run(bt)) DIE("cq_batch_run: %s", cq_last_error());
+        for (int k = 0; k < ns; k++) {
+            shot_buf *b = &sb[k];
+            if (!rtm) {
+                char name[512];
+                float *rec = (float *)malloc(sizeof(float) * (size_t)tmax * (b->nr > 0 ? b->nr : 1));
+                if (cq_batch_read_rec(bt, k, rec)) DIE("cq_batch_read_rec: %s", cq_last_error());
+                snprintf(name, sizeof name, "%s/%s%d_%d.bin", outdir, q ? "recva" : "reca", b->shotid, b->nr);
+                if (cqh_write_f32(name, rec, (long)tmax * b->nr)) DIE("cannot write %s", name);
+                free(rec);
+            } else {
+                /* post-processing of one shot (sgpstd-qrtm/acps4.cu:909-945) */
+                float *img = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                float *imgf = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                float *ssg = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                float *pj = (float *)calloc(nzxt, sizeof(float));
+                if (cq_batch_read_image(bt, k, img, ssg)) DIE("cq_batch_read_image: %s", cq_last_error());
+                cqh_filter_image(nx, nz, dx, dz, img, imgf);
+                cqh_mute_image(nx, nz, pml, b->geo.lgest, b->geo.rgest, imgf);
+                cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, imgf);
+                for (size_t i = 0; i < nzxt; i++) img_sp[i] += pj[i];
+                memset(pj, 0, sizeof(float) * nzxt);
+                cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, ssg);
+                for (size_t i = 0; i < nzxt; i++) ssg_sp[i] += pj[i];
+                free(img); free(imgf); free(ssg); free(pj);
+            }
+            free_shot(b);
+        }
+        cq_batch_destroy(bt);
+        free(sb);
+        free(jd);
+    }
+    const double el = now_s() - t0;
+    printf("The cost of the run time is %f seconds\n", el);
+    if (rtm) {
+        const char *out = q ? "QRTM.BIN" : "RTM.BIN";
+        FILE *fp = fopen(out, "wb");
+        if (!fp) DIE("cannot write %s", out);
+        size_t w1 = fwrite(img_sp, sizeof(float), nzxt, fp);
+        size_t w2 = fwrite(ssg_sp, sizeof(float), nzxt, fp);
+        fclose(fp);
+        if (w1 != nzxt || w2 != nzxt) DIE("short write of %s", out);
+    }
+    cq_close(ctx);
+    return 0;
+}

@@ -473,3 +473,30 @@ int cqh_shot_indices(int pml, int nxx, int nzz, float dx, float dz,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* RTM image post-processing                                                 */
+/* ------------------------------------------------------------------------ */
+
+void cqh_filter_image(int nx, int nz, float dx, float dz, const float *img, float *imgf)
+{
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            long a = (long)ix * nz + iz;
+            if (ix == 0 || iz == 0 || ix == nx - 1 || iz == nz - 1) {
+                imgf[a] = 0.0f;
+                continue;
+            }
+            imgf[a] = (img[a + 1] + img[a - 1] - 2.0 * img[a]) / dz / dz +
+                      (img[a + nz] + img[a - nz] - 2.0 * img[a]) / dx / dx;
+        }
+}
+
+void cqh_mute_image(int nx, int nz, int pml, int lgest, int rgest, float *img)
+{
+    int l = lgest - pml, r = rgest - pml;
+    for (int iz = 0; iz < nz; iz++) {
+        for (int ix = 0; ix < l && ix < nx; ix++) img[(long)ix * nz + iz] = 0.0f;
+        for (int ix = r < 0 ? 0 : r; ix < nx; ix++) img[(long)ix * nz + iz] = 0.0f;
+    }
+}

@@ -64,6 +64,12 @@ typedef struct {
     int xsg, zsg;             /* source grid index */
     int nr;                   /* receivers */
     const int *gxg, *gzg;     /* receiver grid indices, nr each */
+    /* RTM / Q-RTM (kind 2, 3) */
+    const float *obs;         /* observed gather nt*nr, time-major, direct wave
+                               * already removed (removedirect, funca.c:878-899) */
+    int snaps;                /* revolve checkpoints, 1..64 (input file "snapnum") */
+    float etas, etar;         /* Q-RTM stabilisers of the source / receiver
+                               * wavefields, in units of 1e-6 (visac2drtm.txt) */
 } cq_job_desc;
 
 /* Opens device `device` (HIP ordinal) and creates the engine's stream. */
@@ -88,6 +94,12 @@ int cq_batch_run(cq_batch b);
 int cq_batch_read_rec(cq_batch b, int job, float *rec);
 /* Copies shot `job`'s pressure field (nxx*nzz floats) to host memory. */
 int cq_batch_read_p(cq_batch b, int job, float *p);
+/* RTM batches: copies shot `job`'s raw cross-correlation image and source
+ * illumination, nx*nz floats each over the interior (x-major), before the
+ * Laplacian filter, mute and projection of sgpstd-qrtm/acps4.cu:909-945.
+ * An RTM batch holds RTM or Q-RTM shots that share nt and snaps; cq_batch_run
+ * drives the revolve schedule (src.c:314-470) for all of them in lock step. */
+int cq_batch_read_image(cq_batch b, int job, float *img, float *ssg);
 /* Steps taken so far by the batch. */
 int cq_batch_time(cq_batch b);
 /* Device bytes held by the batch. */

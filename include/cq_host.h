@@ -119,6 +119,16 @@ int cqh_shot_indices(int pml, int nxx, int nzz, float dx, float dz,
                      int nr, const float *gxp, const float *gzp,
                      int *gxg, int *gzg, cqh_shot_geom *geo);
 
+/* ---- RTM image post-processing (sgpstd-qrtm/acps4.cu:909-945) ---- */
+
+/* filter_image (acps4.cu:1723-1745): 5-point Laplacian of the raw image over
+ * the interior nx*nz, zero on the border. */
+void cqh_filter_image(int nx, int nz, float dx, float dz, const float *img, float *imgf);
+/* Zero image columns ix < lgest - pml and ix >= rgest - pml (acps4.cu:926-935;
+ * lgest/rgest are padded-grid receiver columns, so the rightmost receiver
+ * column is zeroed too). */
+void cqh_mute_image(int nx, int nz, int pml, int lgest, int rgest, float *img);
+
 #ifdef __cplusplus
 }
 #endif

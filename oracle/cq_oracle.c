@@ -466,6 +466,134 @@ int orc_forward(int kind, int nxx, int nzz, int nt, real dt,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* RTM / Q-RTM driver (sgpstd-qrtm/acps4.cu:447-945, acsglw.cu:470-1079)     */
+/* ------------------------------------------------------------------------ */
+
+static void state_copy(orc_state *dst, const orc_state *src, size_t n)
+{
+    memcpy(dst->p, src->p, sizeof(real) * n);
+    memcpy(dst->vx, src->vx, sizeof(real) * n);
+    memcpy(dst->vz, src->vz, sizeof(real) * n);
+    memcpy(dst->fipx, src->fipx, sizeof(real) * n);
+    memcpy(dst->fipz, src->fipz, sizeof(real) * n);
+    memcpy(dst->fivxx, src->fivxx, sizeof(real) * n);
+    memcpy(dst->fivzz, src->fivzz, sizeof(real) * n);
+    memcpy(dst->aux, src->aux, sizeof(real) * n);
+}
+
+/* One time step of the RTM source (rcv == 0) or receiver (rcv == 1)
+ * wavefield.  q: viscoacoustic with compensation (sign -1) and the stabilised
+ * aux multiplier (eta, it = its time index).  Source: p[src] += rik[t] dt 1000.
+ * Receiver: p = obs[row] at the receiver cells (addsa_b_kernel). */
+static void rtm_step(const orc_model *m, orc_state *s, orc_work *w, int q, int rcv, int t,
+                     const real *rik, size_t src_cell, const int *flagrec, const int *rindex,
+                     const real *obs, int nr, int row, real eta, real taoc)
+{
+    const size_t N = (size_t)m->nxx * m->nzz;
+    step_div(m, s, w);
+    if (!q) {
+        memcpy(w->d1, w->d0, sizeof(cpx) * N);
+        fft2d(&w->px, &w->pz, w->d1, m->nxx, m->nzz, 0);
+        mult_k_ifft(m, w, w->d1, m->k2d1, w->d2);
+        p_update_ac(m, s, w);
+    } else {
+        p_update_q(m, s, w, (real)-1);
+    }
+    if (!rcv) {
+        s->p[src_cell] += rik[t] * m->dt * (real)1000.0;
+    } else {
+        for (size_t a = 0; a < N; a++)
+            if (flagrec[a]) s->p[a] = obs[(size_t)row * nr + rindex[a]];
+    }
+    step_vel(m, s, w);
+    if (q) aux_update(m, s, w, 1, eta, t, taoc);
+}
+
+/*
+ * RTM of one shot following a revolve action trace (act[i], capo[i],
+ * check[i]; actions numbered as revolve.h: 0 advance, 1 takeshot, 2 restore,
+ * 3 firsturn, 4 youturn, 5 terminate).  kind 2: acoustic RTM; 3: Q-RTM.
+ * obs: nt*nr (direct wave removed).  img, ssg: raw interior image and
+ * illumination, nx*nz each (nx = nxx - 2 pml), zeroed here.
+ */
+int orc_rtm(int kind, int nxx, int nzz, int pml, int nt, real dt,
+            const real *const *fields, const real *const *c1d, const real *const *k2,
+            const real *rik, int xsg, int zsg, int nr, const int *gxg, const int *gzg,
+            const real *obs, const int *act, const int *capo, const int *check, int nact, int snaps,
+            real etas, real etar, real *img, real *ssg)
+{
+    orc_model m;
+    orc_state src, rcv, *slot;
+    orc_work w;
+    const int q = (kind == 3);
+    const size_t N = (size_t)nxx * nzz;
+    const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
+    model_from_args(&m, nxx, nzz, dt, fields, c1d, k2);
+    state_alloc(&src, N);
+    state_alloc(&rcv, N);
+    work_init(&w, nxx, nzz);
+    slot = (orc_state *)malloc(sizeof(orc_state) * (size_t)snaps);
+    for (int i = 0; i < snaps; i++) state_alloc(&slot[i], N);
+    int *flagrec = (int *)calloc(N, sizeof(int));
+    int *rindex = (int *)calloc(N, sizeof(int));
+    for (int i = 0; i < nr; i++) {      /* acps4.cu:416-420: last receiver wins */
+        size_t a = (size_t)gxg[i] * nzz + gzg[i];
+        flagrec[a] = 1;
+        rindex[a] = i;
+    }
+    real taoc = 0;
+    if (q)   /* get_max_tao over the padded window (acsglw.cu:354-356) */
+        for (size_t a = 0; a < N; a++)
+            if (m.tao[a] >= taoc) taoc = m.tao[a];
+    memset(img, 0, sizeof(real) * (size_t)nx * nz);
+    memset(ssg, 0, sizeof(real) * (size_t)nx * nz);
+    const size_t src_cell = (size_t)xsg * nzz + zsg;
+    int ts = 0, restart = 0;
+    for (int i = 0; i < nact; i++) {
+        const int a = act[i];
+        if (a == 1) {                    /* takeshot (+ addsource when capo == 0) */
+            state_copy(&slot[check[i]], &src, N);
+            if (capo[i] == 0) slot[check[i]].p[src_cell] += rik[0] * (real)1000.0 * dt;
+        } else if (a == 0) {             /* advance t = ts..capo */
+            for (int t = ts; t <= capo[i]; t++) {
+                rtm_step(&m, &src, &w, q, 0, t, rik, src_cell, NULL, NULL, NULL, 0, 0, etas, taoc);
+                if (restart == 0)        /* illumination, first sweep only */
+                    for (int ix = 0; ix < nx; ix++)
+                        for (int iz = 0; iz < nz; iz++) {
+                            real p = src.p[(size_t)(ix + pml) * nzz + iz + pml];
+                            ssg[(size_t)ix * nz + iz] += p * p;
+                        }
+            }
+            ts = capo[i] + 1;
+        } else if (a == 3 || a == 4) {   /* firsturn / youturn: receiver step + image */
+            if (a == 3) restart = 1;
+            const int row = (a == 3) ? nt - 1 : capo[i];
+            rtm_step(&m, &rcv, &w, q, 1, row, rik, src_cell, flagrec, rindex, obs, nr, row, etar, taoc);
+            for (int ix = 0; ix < nx; ix++)
+                for (int iz = 0; iz < nz; iz++) {
+                    size_t c = (size_t)(ix + pml) * nzz + iz + pml;
+                    img[(size_t)ix * nz + iz] += src.p[c] * rcv.p[c];
+                }
+        } else if (a == 2) {             /* restore */
+            state_copy(&src, &slot[check[i]], N);
+            ts = capo[i] + 1;
+        } else if (a == 5) {
+            break;
+        } else {
+            break;
+        }
+    }
+    for (int i = 0; i < snaps; i++) state_free(&slot[i]);
+    free(slot);
+    free(flagrec);
+    free(rindex);
+    state_free(&src);
+    state_free(&rcv);
+    work_free(&w);
+    return 0;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

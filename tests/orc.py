@@ -104,6 +104,33 @@ def forward(job, bits=64, want_state=False):
     return rec
 
 
+def rtm(job, trace, bits=64):
+    """RTM / Q-RTM of one shot (job kind 2 or 3, with obs, snaps, etas, etar)
+    following the revolve action trace [(action, capo, check, fine), ...].
+    Returns the raw interior (img, ssg), each (nx, nz)."""
+    L = lib(bits)
+    dt_ = _dt(bits)
+    a = _Args(job, bits)
+    nxx, nzz, pml = int(job["nxx"]), int(job["nzz"]), int(job["pml"])
+    nx, nz = nxx - 2 * pml, nzz - 2 * pml
+    obs = np.ascontiguousarray(job["obs"], dt_)
+    act = np.array([x[0] for x in trace], np.int32)
+    capo = np.array([x[1] for x in trace], np.int32)
+    check = np.array([x[2] for x in trace], np.int32)
+    img = np.zeros(nx * nz, dt_)
+    ssg = np.zeros(nx * nz, dt_)
+    real = C.c_double if bits == 64 else C.c_float
+    ip = lambda v: v.ctypes.data_as(C.POINTER(C.c_int))
+    L.orc_rtm(C.c_int(int(job["kind"])), C.c_int(nxx), C.c_int(nzz), C.c_int(pml),
+              C.c_int(int(job["nt"])), real(float(job["dt"])), a.fields, a.c1d, a.k2,
+              _ptr(a.rik, bits), C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])),
+              C.c_int(len(a.gxg)), ip(a.gxg), ip(a.gzg), _ptr(obs, bits), ip(act), ip(capo),
+              ip(check), C.c_int(len(act)), C.c_int(int(job["snaps"])),
+              real(float(job.get("etas", 0.0))), real(float(job.get("etar", 0.0))),
+              _ptr(img, bits), _ptr(ssg, bits))
+    return img.reshape(nx, nz), ssg.reshape(nx, nz)
+
+
 def set_threads(n, bits=64):
     lib(bits).orc_set_threads(C.c_int(n))
 
