@@ -14,7 +14,9 @@
 #ifndef CQ_NT
 #define CQ_NT 256
 #endif
+#ifndef CQ_MAXP
 #define CQ_MAXP 34       // complex values a thread holds per stage (2 radix-17 butterflies)
+#endif
 #define CQ_MAXST 12      // max stages per plan
 
 struct CqPlan {
@@ -215,6 +217,9 @@ __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
     const int m = n / R;
     const int total = L * m;
     const int twmul = n / (Ns * R);
+    // exact quotients by float reciprocal: operands are below 2^14, so
+    // (i + 0.5) / d never lies within rounding distance of an integer
+    const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
     float2 v[NB][R];
     int dst[NB];
 #pragma unroll
@@ -222,9 +227,9 @@ __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
         const int idx = threadIdx.x + b * CQ_NT;
         dst[b] = -1;
         if (idx < total) {
-            const int l = idx / m;
+            const int l = (int)(((float)idx + 0.5f) * inv_m);
             const int j = idx - l * m;
-            const int k = j % Ns;
+            const int k = j - (int)(((float)j + 0.5f) * inv_ns) * Ns;
             const float2 *src = buf + l * LS + j;
             const int tstep = k * twmul;
 #pragma unroll
