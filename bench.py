@@ -37,12 +37,15 @@ def algorithmic_bytes(jobs):
       dvx (40 B/cell) + x-CPML strips (phi, psi read+write, 16 B per strip cell).
     Acoustic column pass: reads p, vz, dt*bz, dvx; writes vz, div, Lz
       (28 B/cell) + z-CPML strips (16 B per strip cell).
-    Viscoacoustic: C1 reads p, vz, dt*bz, dvx, writes vz, div (24 B/cell) and
-      2 half spectra; R1 reads 2, writes 4 half spectra; C2 reads 3 half
-      spectra, div, gam, y2, c (16 B/cell), writes total (4 B/cell) and 1
-      half spectrum; R2 reads 1, writes 1; C3 reads 2 half spectra, p, total,
-      3 coefficients (24 B/cell), writes p (4 B/cell); R3 reads p, vx, dt*bx,
-      writes vx, dvx (20 B/cell) + strips."""
+    Viscoacoustic (one half spectrum = nxx*(nzz/2+1) complex words):
+      C1 reads div (4 B/cell), writes 1 half spectrum;
+      R1 reads 2, writes 4 half spectra;
+      C2 reads 3 half spectra, div, gam, y2, c, writes total (20 B/cell) and
+        1 half spectrum;
+      R2 reads 1, writes 1 half spectrum;
+      C3 reads 2 half spectra, p, total, 3 coefficients, vz, dt*bz, writes p,
+        vz, dvz (40 B/cell), 1 half spectrum + z-CPML strips;
+      R3 reads p, vx, dt*bx, dvz, writes vx, div (24 B/cell) + x-CPML strips."""
     out = {}
 
     def add(k, v):
@@ -57,12 +60,12 @@ def algorithmic_bytes(jobs):
             add("k_row_ac", 40 * n + sx)
             add("k_col_ac", 28 * n + sz)
         else:
-            add("k_q_c1", 24 * n + 2 * hs + sz)
+            add("k_q_c1", 4 * n + hs)
             add("k_q_r1", 6 * hs)
             add("k_q_c2", 20 * n + 4 * hs)
             add("k_q_r2", 2 * hs)
-            add("k_q_c3", 28 * n + 2 * hs)
-            add("k_q_r3", 20 * n + sx)
+            add("k_q_c3", 40 * n + 3 * hs + sz)
+            add("k_q_r3", 24 * n + sx)
     return out
 
 

@@ -76,54 +76,54 @@ __global__ __launch_bounds__(256) void k_copy_f4(const float4 *__restrict__ a, f
 struct cq_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::map<int, std::pair<CqPlan, float2 *>> plans;  // n -> plan, device twiddles
+    std::map<int, float2 *> twiddles;  // n -> device table w_n^j, j < n
     ~cq_ctx_s()
     {
-        for (auto &kv : plans) (void)hipFree(kv.second.second);
+        for (auto &kv : twiddles) (void)hipFree(kv.second);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
-// Factorisation into supported radices: powers of two as 16/8/4/2, powers of
-// three as 9/3, then 5, 7, 11, 13, 17.
-static bool make_plan(int n, CqPlan &p)
+// Factorisation of n into supported radices for a workgroup that transforms
+// `need` points at once: the fewest stages, taking powers of two as 16/8/4/2
+// and powers of three as 9/3, then 5, 7, 11, 13, 17, but only radices whose
+// per-stage capacity (cq_stage_capacity) holds `need` points.  False if n
+// has another prime factor or a prime factor's radix is too small.
+static bool make_plan(int n, int need, CqPlan &p)
 {
     memset(&p, 0, sizeof p);
     p.n = n;
     int m = n;
-    while (m % 16 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 16; m /= 16; }
-    if (m % 8 == 0) { p.rad[p.ns++] = 8; m /= 8; }
-    if (m % 4 == 0) { p.rad[p.ns++] = 4; m /= 4; }
-    if (m % 2 == 0) { p.rad[p.ns++] = 2; m /= 2; }
-    while (m % 9 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 9; m /= 9; }
-    while (m % 3 == 0 && p.ns < CQ_MAXST) { p.rad[p.ns++] = 3; m /= 3; }
-    for (int r : {5, 7, 11, 13, 17})
-        while (m % r == 0) {
-            if (p.ns >= CQ_MAXST) return false;
-            p.rad[p.ns++] = r;
-            m /= r;
-        }
-    return m == 1 && p.ns <= CQ_MAXST;
+    auto push = [&](int r) {
+        if (p.ns >= CQ_MAXST) return false;
+        p.rad[p.ns++] = r;
+        m /= r;
+        return true;
+    };
+    for (int r : {16, 8, 4, 2, 9, 3, 5, 7, 11, 13, 17}) {
+        if (cq_stage_capacity(r) < need) continue;
+        while (m % r == 0)
+            if (!push(r)) return false;
+    }
+    return m == 1;
 }
 
-static int plan_capacity(const CqPlan &p)
+static bool has_plan(int n, int need)
 {
-    int cap = 1 << 30;
-    for (int s = 0; s < p.ns; s++) cap = std::min(cap, cq_stage_capacity(p.rad[s]));
-    return cap;
+    CqPlan p;
+    return make_plan(n, need, p);
 }
 
-static int get_plan(cq_ctx ctx, int n, CqPlan &p, const float2 *&tw)
+static int get_plan(cq_ctx ctx, int n, int need, CqPlan &p, const float2 *&tw)
 {
-    auto itp = ctx->plans.find(n);
-    if (itp != ctx->plans.end()) {
-        p = itp->second.first;
-        tw = itp->second.second;
+    if (!make_plan(n, need, p))
+        return set_err(CQ_EUNSUPPORTED,
+                       "FFT length %d: no factorisation into radices 2..17 for %d points per workgroup", n, need);
+    auto itp = ctx->twiddles.find(n);
+    if (itp != ctx->twiddles.end()) {
+        tw = itp->second;
         return CQ_OK;
     }
-    if (!make_plan(n, p))
-        return set_err(CQ_EUNSUPPORTED,
-                       "FFT length %d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", n);
     std::vector<float2> h(n);
     for (int j = 0; j < n; j++) {
         const double a = -2.0 * M_PI * (double)j / (double)n;
@@ -132,7 +132,7 @@ static int get_plan(cq_ctx ctx, int n, CqPlan &p, const float2 *&tw)
     float2 *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(float2) * n));
     HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    ctx->plans[n] = {p, d};
+    ctx->twiddles[n] = d;
     tw = d;
     return CQ_OK;
 }
@@ -254,7 +254,7 @@ struct cq_batch_s {
     int2 *ditems[2][P_COUNT] = {};
     int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
-    int tz = 16, cw = 16, tk = 8;
+    int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
     int t = 0, tmax = 0;
     // RTM schedule state
     int snaps = 0, rtm_done = 0, rv_info = 0;
@@ -351,7 +351,7 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
 static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
 {
     cq_ctx ctx = b->ctx;
-    const int cw = b->cw, tk = b->tk, tz = b->tz;
+    const int cw = is_q(j.kind) ? b->cwq : b->cw, tk = b->tk, tz = b->tz;
     const bool q = is_q(j.kind), rtm = is_rtm(j.kind);
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = (nzz + 31) & ~31;
@@ -377,8 +377,11 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
     base.stab = (j.kind == CQ_KIND_QRTM) ? 1 : 0;
     base.gather = rtm ? 0 : 1;
     int rc;
-    if ((rc = get_plan(ctx, nxx, base.px, base.twx)) != CQ_OK) return rc;
-    if ((rc = get_plan(ctx, nzz, base.pz, base.twz)) != CQ_OK) return rc;
+    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles up to
+    // 3 tk lines of nxx (k_q_r1); column tiles cw/2 lines of nzz
+    const int need_x = std::max((tz / 2) * nxx, q ? 3 * tk * nxx : 0), need_z = (cw / 2) * nzz;
+    if ((rc = get_plan(ctx, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
+    if ((rc = get_plan(ctx, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
 
     const StateLayout sl = state_layout(nxx, ldz, ldk, pml, q);
     Arena A;
@@ -595,6 +598,23 @@ static std::vector<int2> xcd_order(const std::vector<int2> &items)
 }
 
 static const size_t kLdsMax = 160 * 1024;
+// LDS budget per workgroup that the tile choice aims for: 80 KB lets two
+// workgroups share a CU; tiles fall back to kLdsMax when a window needs it.
+#ifndef CQ_LDS_TILE
+#define CQ_LDS_TILE (80 * 1024)
+#endif
+static const size_t kLdsTile = CQ_LDS_TILE;
+
+// LDS bytes of a workgroup holding the twiddle table of an n-point transform
+// and `lines` padded lines of it.
+static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_stride(n)) * 8; }
+// ... plus the 16-byte aligned per-column float4 accumulator of a cw-column
+// tile (col_acc).
+static size_t lds_col_acc(int n, int cw)
+{
+    const int off = n + (cw / 2) * cq_line_stride(n);
+    return (size_t)((off + 1) & ~1) * 8 + (size_t)cw * ((n + 3) / 4) * 16;
+}
 
 extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out)
 {
@@ -615,31 +635,37 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     }
     // Tile shapes: the largest tiles whose lines fit one workgroup's per-stage
     // capacity and the LDS for every shot of the batch.
-    int tz = 16, cw = 32, tk = 8;
+    int tz = 16, cw = 32, cwq = 32, tk = 8;
     int maxnxx = 0, maxnzz = 0;
     for (int i = 0; i < njobs; i++) {
         const cq_job_desc &j = jobs[i];
-        CqPlan px, pz;
-        if (!make_plan(j.nxx, px))
+        if (!has_plan(j.nxx, 1))
             return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nxx);
-        if (!make_plan(j.nzz, pz))
+        if (!has_plan(j.nzz, 1))
             return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nzz);
         const bool q = is_q(j.kind);
-        while (tz > 8 && ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax)) tz /= 2;
-        while (cw > 2 && ((cw / 2) * j.nzz > plan_capacity(pz) ||
-                          (size_t)(1 + cw / 2) * j.nzz * 8 + (q ? (size_t)cw * j.nzz * 4 : 0) > kLdsMax))
-            cw /= 2;
+        auto row_bad = [&](int t, size_t lim) {
+            return !has_plan(j.nxx, (t / 2) * j.nxx) || lds_lines(j.nxx, t / 2) > lim;
+        };
+        auto col_bad = [&](int c, size_t lim) {
+            return !has_plan(j.nzz, (c / 2) * j.nzz) || (q ? lds_col_acc(j.nzz, c) : lds_lines(j.nzz, c / 2)) > lim;
+        };
+        auto krow_bad = [&](int k, size_t lim) {
+            return !has_plan(j.nxx, 3 * k * j.nxx) || lds_lines(j.nxx, 3 * k) > lim;
+        };
+        int &cwk = q ? cwq : cw;
+        while (tz > 8 && row_bad(tz, kLdsTile)) tz /= 2;
+        while (cwk > 2 && col_bad(cwk, kLdsTile)) cwk /= 2;
         if (q)
-            while (tk > 2 && (tk * j.nxx > plan_capacity(px) || (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)) tk /= 2;
-        if ((tz / 2) * j.nxx > plan_capacity(px) || (size_t)(1 + tz / 2) * j.nxx * 8 > kLdsMax ||
-            (cw / 2) * j.nzz > plan_capacity(pz) ||
-            (q && (tk * j.nxx > plan_capacity(px) || (size_t)(1 + 2 * tk) * j.nxx * 8 > kLdsMax)))
+            while (tk > 2 && krow_bad(tk, kLdsTile)) tk /= 2;
+        if (row_bad(tz, kLdsMax) || col_bad(cwk, kLdsMax) || (q && krow_bad(tk, kLdsMax)))
             return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", i, j.nxx, j.nzz);
         maxnxx = std::max(maxnxx, j.nxx);
         maxnzz = std::max(maxnzz, j.nzz);
     }
     b->tz = tz;
     b->cw = cw;
+    b->cwq = cwq;
     b->tk = tk;
     b->snaps = b->rtm ? jobs[0].snaps : 0;
     b->jobs.resize(njobs);
@@ -655,7 +681,7 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
                 for (int z0 = 0; z0 < h.nzz; z0 += tz) it[P_AROW].push_back(make_int2(i, z0));
                 for (int x0 = 0; x0 < h.nxx; x0 += cw) it[P_ACOL].push_back(make_int2(i, x0));
             } else {
-                for (int x0 = 0; x0 < h.nxx; x0 += cw) {
+                for (int x0 = 0; x0 < h.nxx; x0 += cwq) {
                     it[P_QC1].push_back(make_int2(i, x0));
                     it[P_QC2].push_back(make_int2(i, x0));
                     it[P_QC3].push_back(make_int2(i, x0));
@@ -668,16 +694,16 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
             }
         }
     }
-    const size_t lrow = (size_t)(1 + tz / 2) * maxnxx * 8;
-    const size_t lcol = (size_t)(1 + cw / 2) * maxnzz * 8;
+    const size_t lrow = lds_lines(maxnxx, tz / 2);
+    const size_t lcol = lds_lines(maxnzz, cw / 2);
     b->lds[P_AROW] = lrow;
     b->lds[P_QR3] = lrow;
     b->lds[P_ACOL] = lcol;
-    b->lds[P_QC1] = lcol;
-    b->lds[P_QC2] = lcol + (size_t)cw * maxnzz * 4;
-    b->lds[P_QC3] = lcol + (size_t)cw * maxnzz * 4;
-    b->lds[P_QR1] = (size_t)(1 + 2 * tk) * maxnxx * 8;
-    b->lds[P_QR2] = (size_t)(1 + tk) * maxnxx * 8;
+    b->lds[P_QC1] = lds_lines(maxnzz, cwq / 2);
+    b->lds[P_QC2] = lds_col_acc(maxnzz, cwq);
+    b->lds[P_QC3] = lds_col_acc(maxnzz, cwq);
+    b->lds[P_QR1] = lds_lines(maxnxx, 3 * tk);
+    b->lds[P_QR2] = lds_lines(maxnxx, tk);
     for (int p = 0; p < P_COUNT; p++)
         if (!items[0][p].empty() && b->lds[p] > kLdsMax)
             return set_err(CQ_EUNSUPPORTED, "pass %s needs %zu B of LDS", kPassName[p], b->lds[p]);
@@ -717,12 +743,12 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step, int t_aux,
 #define CQ_ARGS pass, role, step, t_aux, flags
 #define CQ_DISPATCH_TZ(K)                                                          \
     (b->tz == 16 ? launch(b, K<16>, CQ_ARGS) : launch(b, K<8>, CQ_ARGS))
-#define CQ_DISPATCH_CW(K)                                                          \
-    (b->cw == 32   ? launch(b, K<32>, CQ_ARGS)                                     \
-     : b->cw == 16 ? launch(b, K<16>, CQ_ARGS)                                     \
-     : b->cw == 8  ? launch(b, K<8>, CQ_ARGS)                                      \
-     : b->cw == 4  ? launch(b, K<4>, CQ_ARGS)                                      \
-                   : launch(b, K<2>, CQ_ARGS))
+#define CQ_DISPATCH_CW(K, W)                                                       \
+    (W == 32   ? launch(b, K<32>, CQ_ARGS)                                         \
+     : W == 16 ? launch(b, K<16>, CQ_ARGS)                                         \
+     : W == 8  ? launch(b, K<8>, CQ_ARGS)                                          \
+     : W == 4  ? launch(b, K<4>, CQ_ARGS)                                          \
+               : launch(b, K<2>, CQ_ARGS))
 #define CQ_DISPATCH_TK(K)                                                          \
     (b->tk == 8 ? launch(b, K<8>, CQ_ARGS)                                         \
      : b->tk == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
@@ -731,12 +757,12 @@ static int launch_pass(cq_batch b, int pass, int role, int step, int t_aux, int 
 {
     switch (pass) {
     case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
-    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac);
-    case P_QC1: return CQ_DISPATCH_CW(k_q_c1);
+    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, b->cw);
+    case P_QC1: return CQ_DISPATCH_CW(k_q_c1, b->cwq);
     case P_QR1: return CQ_DISPATCH_TK(k_q_r1);
-    case P_QC2: return CQ_DISPATCH_CW(k_q_c2);
+    case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
     case P_QR2: return CQ_DISPATCH_TK(k_q_r2);
-    case P_QC3: return CQ_DISPATCH_CW(k_q_c3);
+    case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
     case P_QR3: return CQ_DISPATCH_TZ(k_q_r3);
     }
     return set_err(CQ_EINVAL, "unknown pass %d", pass);

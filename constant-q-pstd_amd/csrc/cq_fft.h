@@ -11,19 +11,34 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Workgroups of 512 threads.  A thread holds the butterflies it computes in
+// one stage in registers (cq_nb(R) of them: 16-18 complex values, 32 for
+// radices 7/8 whose butterflies need few temporaries), which keeps every pass
+// at <= 128 VGPRs, i.e. 4 waves per SIMD when two workgroups share a CU.
 #ifndef CQ_NT
-#define CQ_NT 256
-#endif
-#ifndef CQ_MAXP
-#define CQ_MAXP 34       // complex values a thread holds per stage (2 radix-17 butterflies)
+#define CQ_NT 512
 #endif
 #define CQ_MAXST 12      // max stages per plan
+
+// butterflies per thread per stage for radix R
+__host__ __device__ constexpr int cq_nb(int R)
+{
+    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : 1;
+}
 
 struct CqPlan {
     int n;
     int ns;
     int rad[CQ_MAXST];
 };
+
+// LDS line layout: value i of a line sits at slot cq_pad(i), one unused slot
+// after every 16 values, so that the stride-R stores of the early Stockham
+// stages (and stride-4 accesses of the column helpers) spread over the LDS
+// banks instead of landing on one.  Lines are cq_line_stride(n) slots apart
+// (odd, so consecutive lines start on different banks).
+__host__ __device__ constexpr int cq_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int cq_line_stride(int n) { return cq_pad(n) | 1; }
 
 // ---------------------------------------------------------------------------
 // compile-time trigonometry for the butterfly constants
@@ -213,7 +228,7 @@ template <int R, bool INV>
 __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
                                          const float2 *tw, const float2 *__restrict__ mh)
 {
-    constexpr int NB = (CQ_MAXP / R) > 0 ? (CQ_MAXP / R) : 1;
+    constexpr int NB = cq_nb(R);
     const int m = n / R;
     const int total = L * m;
     const int twmul = n / (Ns * R);
@@ -221,20 +236,21 @@ __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
     // (i + 0.5) / d never lies within rounding distance of an integer
     const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
     float2 v[NB][R];
-    int dst[NB];
+    int base[NB], kk[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int idx = threadIdx.x + b * CQ_NT;
-        dst[b] = -1;
+        base[b] = -1;
+        kk[b] = 0;
         if (idx < total) {
             const int l = (int)(((float)idx + 0.5f) * inv_m);
             const int j = idx - l * m;
             const int k = j - (int)(((float)j + 0.5f) * inv_ns) * Ns;
-            const float2 *src = buf + l * LS + j;
+            const float2 *src = buf + l * LS;
             const int tstep = k * twmul;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                float2 x = src[r * m];
+                float2 x = src[cq_pad(j + r * m)];
                 if (r > 0 && tstep != 0) {
                     const float2 w = tw[r * tstep];
                     x = INV ? cmulc(x, w) : cmul(x, w);
@@ -242,20 +258,20 @@ __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
                 v[b][r] = x;
             }
             Dft<R, INV>::run(v[b]);
-            dst[b] = l * LS + (j - k) * R + k;
+            base[b] = l * LS;
+            kk[b] = (j - k) * R + k;
             if (mh != nullptr) {
-                const int kk = (j - k) * R + k;
 #pragma unroll
-                for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk + q * Ns]);
+                for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk[b] + q * Ns]);
             }
         }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        if (dst[b] >= 0) {
+        if (base[b] >= 0) {
 #pragma unroll
-            for (int q = 0; q < R; q++) buf[dst[b] + q * Ns] = v[b][q];
+            for (int q = 0; q < R; q++) buf[base[b] + cq_pad(kk[b] + q * Ns)] = v[b][q];
         }
     }
     __syncthreads();
@@ -308,5 +324,5 @@ __device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS,
 // a given radix (host and device agree on this).
 __host__ __device__ constexpr int cq_stage_capacity(int R)
 {
-    return CQ_NT * ((CQ_MAXP / R) > 0 ? (CQ_MAXP / R) : 1) * R;
+    return CQ_NT * cq_nb(R) * R;
 }

@@ -37,11 +37,12 @@ __device__ __forceinline__ void load_tw(float2 *tw, const float2 *__restrict__ g
 template <int Q>
 __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float *__restrict__ src, int ldz, int z0)
 {
+    const int LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int ix = e / Q, q = e % Q;
         const float4 d = ld4(src + (size_t)ix * ldz + z0 + 4 * q);
-        buf[(2 * q) * nxx + ix] = make_float2(d.x, d.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(d.z, d.w);
+        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(d.x, d.y);
+        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(d.z, d.w);
     }
 }
 
@@ -50,11 +51,11 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float *__re
 template <int Q>
 __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
 {
-    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml;
+    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int ix = e / Q, q = e % Q;
         const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         float4 dp = make_float4(a.x, a.y, b.x, b.y);
         const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
         if (ix < pml || ix >= nxx - pml) {
@@ -75,8 +76,8 @@ __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
         vn.z = vx.z + bx.z * dp.z;
         vn.w = vx.w + bx.w * dp.w;
         st4(J.vx + g, vn);
-        buf[(2 * q) * nxx + ix] = make_float2(vn.x, vn.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(vn.z, vn.w);
+        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(vn.x, vn.y);
+        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(vn.z, vn.w);
     }
 }
 
@@ -86,11 +87,11 @@ __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
 template <int Q, bool TO_DIV = false>
 __device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, int z0)
 {
-    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml;
+    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int ix = e / Q, q = e % Q;
         const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         float4 dv = make_float4(a.x, a.y, b.x, b.y);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
@@ -119,7 +120,16 @@ __device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, 
 // ===========================================================================
 __device__ __forceinline__ float *col_line(float *bufs, int nzz, int c)
 {
-    return bufs + 2 * (c >> 1) * nzz + (c & 1);
+    return bufs + 2 * (c >> 1) * cq_line_stride(nzz) + (c & 1);
+}
+
+// Per-column accumulator of a column tile: one float4 per (column c, depths
+// 4 zq .. 4 zq + 3) at index c * nq + zq, placed 16-byte aligned after the
+// twiddle table and the L packed lines.
+__device__ __forceinline__ float4 *col_acc(float2 *lds, int nzz, int L)
+{
+    const int off = nzz + L * cq_line_stride(nzz);
+    return reinterpret_cast<float4 *>(lds + ((off + 1) & ~1));
 }
 
 template <int CW>
@@ -133,7 +143,7 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobD &J, const float
         float *line = col_line(bufs, nzz, c);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
+            if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = f4get(v, k);
     }
 }
 
@@ -164,7 +174,7 @@ __device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float dp = line[2 * iz];
+            float dp = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
                 float *php = J.phiz + (size_t)ix * (2 * pml) + sz;
@@ -174,7 +184,7 @@ __device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
             }
             const float v = f4get(vz, k) + f4get(bz, k) * dp;
             f4set(vn, k, v);
-            line[2 * iz] = v;
+            line[2 * cq_pad(iz)] = v;
         }
         st4(J.vz + g, vn);
     }
@@ -193,7 +203,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
         if (ix >= J.nxx) {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
             continue;
         }
         const size_t g = (size_t)ix * J.ldz + 4 * zq;
@@ -203,7 +213,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float d = line[2 * iz];
+            float d = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
                 float *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
@@ -213,7 +223,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
             }
             const float s = f4get(dvx, k) + d;
             f4set(dv, k, s);
-            line[2 * iz] = s;
+            line[2 * cq_pad(iz)] = s;
         }
         st4((TO_DVZ ? J.dvz : J.div) + g, dv);
     }
@@ -232,16 +242,16 @@ __device__ __forceinline__ void col_extract(const JobD &J, float2 *buf, int x0, 
     for (int e = threadIdx.x; e < L * nzh; e += CQ_NT) {
         const int l = e / nzh, k = e - l * nzh;
         const int k2 = k ? nzz - k : 0;
-        float2 *line = buf + l * nzz;
-        const float2 zk = line[k], zm = line[k2];
+        float2 *line = buf + l * cq_line_stride(nzz);
+        const float2 zk = line[cq_pad(k)], zm = line[cq_pad(k2)];
         const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
         const int ix0 = x0 + 2 * l;
         if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
         if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
         if (mh != nullptr) {
-            line[k] = cmul(zk, mh[k]);
-            if (k2 != k) line[k2] = cmul(zm, mh[k2]);
+            line[cq_pad(k)] = cmul(zk, mh[k]);
+            if (k2 != k) line[cq_pad(k2)] = cmul(zm, mh[k2]);
         }
     }
 }
@@ -261,13 +271,13 @@ __device__ __forceinline__ void col_insert(const JobD &J, float2 *buf, int x0, c
         float2 h0 = make_float2(0.f, 0.f), h1 = make_float2(0.f, 0.f);
         if (ix0 < J.nxx) h0 = S[(size_t)ix0 * ldk + k];
         if (ix0 + 1 < J.nxx) h1 = S[(size_t)(ix0 + 1) * ldk + k];
-        float2 *line = buf + l * nzz;
+        float2 *line = buf + l * cq_line_stride(nzz);
         if (k2 == k) {
             h0.y = 0.f;
             h1.y = 0.f;
         }
-        line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
-        if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
+        line[cq_pad(k)] = make_float2(h0.x - h1.y, h0.y + h1.x);
+        if (k2 != k) line[cq_pad(k2)] = make_float2(h0.x + h1.y, h1.x - h0.y);
     }
 }
 
@@ -278,12 +288,12 @@ template <int TK>
 __device__ __forceinline__ void krow_fill(float2 *buf, const JobD &J, const float2 *__restrict__ S, int k0)
 {
     constexpr int H = TK / 2;
-    const int nxx = J.nxx;
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
         const float4 v = *reinterpret_cast<const float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
-        buf[(2 * h) * nxx + ix] = make_float2(v.x, v.y);
-        buf[(2 * h + 1) * nxx + ix] = make_float2(v.z, v.w);
+        buf[(2 * h) * LS + cq_pad(ix)] = make_float2(v.x, v.y);
+        buf[(2 * h + 1) * LS + cq_pad(ix)] = make_float2(v.z, v.w);
     }
 }
 
@@ -291,10 +301,10 @@ template <int TK>
 __device__ __forceinline__ void krow_store(const float2 *buf, const JobD &J, float2 *__restrict__ S, int k0)
 {
     constexpr int H = TK / 2;
-    const int nxx = J.nxx;
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
-        const float2 a = buf[(2 * h) * nxx + ix], b = buf[(2 * h + 1) * nxx + ix];
+        const float2 a = buf[(2 * h) * LS + cq_pad(ix)], b = buf[(2 * h + 1) * LS + cq_pad(ix)];
         *reinterpret_cast<float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
     }
 }
@@ -314,19 +324,39 @@ __device__ __forceinline__ float frac_symbol(int which, float k2)
     return which == 0 ? r : k2 * r;
 }
 
-// buf[l][kx] = (save ? copy : buf)[l][kx] * symbol(kx, k0 + l) / N; with
-// save != null the unmultiplied values are first copied there.
+// The three dispersion symbols at once: spectrum line l (of TK) becomes
+// line l x |k|^(1/16) / N, line TK + l x |k|^2 / N and line 2 TK + l x
+// |k|^(33/16) / N (the frac_symbol values 0, 1, 2).
+template <int TK>
+__device__ __forceinline__ void krow_mult3(const JobD &J, float2 *buf, int k0)
+{
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
+        const float fkx = (float)ikx * J.dkx;
+        const float fkz = (float)(k0 + l) * J.dkz;
+        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
+        const float r = exp2f(log2f(k2) * (1.0f / 32.0f));
+        const float ma = r * J.invN, mk = k2 * J.invN, mb = (k2 * r) * J.invN;
+        const float2 v = buf[pe];
+        buf[pe] = make_float2(v.x * ma, v.y * ma);
+        buf[TK * LS + pe] = make_float2(v.x * mk, v.y * mk);
+        buf[2 * TK * LS + pe] = make_float2(v.x * mb, v.y * mb);
+    }
+}
+
+// buf[l][kx] *= symbol(kx, k0 + l) / N.
 // which = 4: the stabilised compensation multiplier of Q-RTM,
 //   S = k - 2 e k (1 + tc dt k) / (exp(-tc tt k) + e exp(tc dt k)),  k = |k|^1.01,
 // e = eta * 1e-6, tc = max tao, tt = t_aux dt (multt_kernel,
 // sgpstd-qrtm/acsglw.cu:1895-1924).
 template <int TK>
-__device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *save, const float2 *from, int k0,
-                                          int which, float tt = 0.f)
+__device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, int k0, int which, float tt = 0.f)
 {
-    const int nxx = J.nxx;
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx;
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
         const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
         const float fkx = (float)ikx * J.dkx;
         const float fkz = (float)(k0 + l) * J.dkz;
@@ -341,9 +371,8 @@ __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, float2 *sa
             m = frac_symbol(which, k2);
         }
         m *= J.invN;
-        float2 v = from ? from[e] : buf[e];
-        if (save) save[e] = v;
-        buf[e] = make_float2(v.x * m, v.y * m);
+        const float2 v = buf[pe];
+        buf[pe] = make_float2(v.x * m, v.y * m);
     }
 }
 
@@ -383,14 +412,14 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y;
+    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y, LS = cq_line_stride(nxx);
     extern __shared__ float2 lds[];
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.div, ldz, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mx2);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
+    cq_fft_fwd(J.px, buf, LS, L, tw, J.mx2);
+    cq_fft_inv(J.px, buf, LS, L, tw);
 
     // p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498);
     // illumination ssg += p^2 during the first forward sweep of RTM
@@ -400,7 +429,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int ix = e / Q, q = e % Q;
         const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-        const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g), lz = ld4(J.lz + g),
                      d = ld4(J.div + g);
         float4 pn;
@@ -416,8 +445,8 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
             st4(J.p + g, pn);
             if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, z0 + 4 * q, pn, pn);
         }
-        buf[(2 * q) * nxx + ix] = make_float2(pn.x, pn.y);
-        buf[(2 * q + 1) * nxx + ix] = make_float2(pn.z, pn.w);
+        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(pn.x, pn.y);
+        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(pn.z, pn.w);
     }
     __syncthreads();
     if (rcv) {
@@ -426,26 +455,26 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
         const int tile = z0 / TZ;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT) {
             const int dz = J.inj_z[r] - z0;
-            float *f = reinterpret_cast<float *>(buf + (dz >> 1) * nxx + J.inj_x[r]) + (dz & 1);
+            float *f = reinterpret_cast<float *>(buf + (dz >> 1) * LS + cq_pad(J.inj_x[r])) + (dz & 1);
             *f = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         }
         __syncthreads();
         for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
             const int ix = e / Q, q = e % Q;
             const size_t g = (size_t)ix * ldz + z0 + 4 * q;
-            const float2 a = buf[(2 * q) * nxx + ix], b = buf[(2 * q + 1) * nxx + ix];
+            const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
             const float4 pn = make_float4(a.x, a.y, b.x, b.y);
             st4(J.p + g, pn);
             interior_accum(J, J.img, ix, z0 + 4 * q, pn, ld4(J.psrc + g));
         }
         __syncthreads();
     }
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
+    cq_fft_fwd(J.px, buf, LS, L, tw, J.mxp);
+    cq_fft_inv(J.px, buf, LS, L, tw);
     row_vx_phase<Q>(J, buf, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
+    cq_fft_fwd(J.px, buf, LS, L, tw, J.mxm);
+    cq_fft_inv(J.px, buf, LS, L, tw);
     row_dvx_phase<Q>(J, buf, z0);
 }
 
@@ -464,17 +493,17 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
     col_fill<CW>(bufs, J, J.p, x0);
     __syncthreads();
     // receiver gather of the updated p (acps4.cu:611-621)
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzp);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * cq_pad(z)]; });
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzp);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     col_div_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mz2);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mz2);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
@@ -482,7 +511,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) f4set(o, k, line[2 * (4 * zq + k)]);
+            if (4 * zq + k < nzz) f4set(o, k, line[2 * cq_pad(4 * zq + k)]);
         st4(J.lz + (size_t)ix * J.ldz + 4 * zq, o);
     }
 }
@@ -513,7 +542,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
     load_tw(tw, J.twz, nzz);
     col_fill<CW>(bufs, J, J.div, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -522,29 +551,29 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
                                                 int step, int t_aux, int flags)
 {
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, k0 = it.y;
+    const int nxx = J.nxx, k0 = it.y, LS = cq_line_stride(nxx);
     extern __shared__ float2 lds[];
-    float2 *tw = lds, *buf = lds + nxx, *sav = buf + TK * nxx;
+    float2 *tw = lds, *buf = lds + nxx;   // 3 TK lines: A, K, B
     load_tw(tw, J.twx, nxx);
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
-    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566)
-    float2 *outs[3] = {J.s1, J.s3, J.s4};
-    for (int w = 0; w < 3; w++) {
-        krow_mult<TK>(J, buf, w == 0 ? sav : nullptr, w == 0 ? nullptr : sav, k0, w);
-        __syncthreads();
-        cq_fft_inv(J.px, buf, nxx, TK, tw);
-        krow_store<TK>(buf, J, outs[w], k0);
-        __syncthreads();
-    }
+    cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
+    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566),
+    // inverse-transformed together
+    krow_mult3<TK>(J, buf, k0);
+    __syncthreads();
+    cq_fft_inv(J.px, buf, LS, 3 * TK, tw);
+    krow_store<TK>(buf, J, J.s1, k0);
+    krow_store<TK>(buf + TK * LS, J, J.s3, k0);
+    krow_store<TK>(buf + 2 * TK * LS, J, J.s4, k0);
+    __syncthreads();
     // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
     krow_fill<TK>(buf, J, J.s2, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
-    krow_mult<TK>(J, buf, nullptr, nullptr, k0, J.stab ? 4 : 3, (float)t_aux * J.dt);
+    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
+    krow_mult<TK>(J, buf, k0, J.stab ? 4 : 3, (float)t_aux * J.dt);
     __syncthreads();
-    cq_fft_inv(J.px, buf, nxx, TK, tw);
+    cq_fft_inv(J.px, buf, cq_line_stride(nxx), TK, tw);
     krow_store<TK>(buf, J, J.s2, k0);
 }
 
@@ -558,70 +587,73 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     extern __shared__ float2 lds[];
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
-    float *acc = reinterpret_cast<float *>(buf + L * nzz);
+    float4 *acc = col_acc(lds, nzz, L);
     load_tw(tw, J.twz, nzz);
     // total = y1 div + y2 A - (dt^2 v^2 lam/12)(y3 K + y4 B), with
     // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
         const float4 g = ld4(J.qg + (size_t)ix * ldz + 4 * zq);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) acc[c * nzz + iz] = (1.0f - 64.0f * f4get(g, k)) * line[2 * iz];
+            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(g, k)) * line[2 * cq_pad(iz)]);
         }
+        acc[e] = a;
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s4);   // B
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
         const size_t g = (size_t)ix * ldz + 4 * zq;
         const float4 y2 = ld4(J.qy2 + g), cc = ld4(J.qcc + g);
+        float4 a = acc[e];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz)
-                acc[c * nzz + iz] = f4get(cc, k) * (acc[c * nzz + iz] + 2.0f * f4get(y2, k) * line[2 * iz]);
+            if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * cq_pad(iz)]));
         }
+        acc[e] = a;
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s1);   // A
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         float *line = col_line(bufs, nzz, c);
         if (ix >= J.nxx) {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
             continue;
         }
         const size_t g = (size_t)ix * ldz + 4 * zq;
-        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g);
+        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g), a = acc[e];
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * iz] -
-                             acc[c * nzz + iz];
+            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * cq_pad(iz)] -
+                             f4get(a, k);
             f4set(o, k, tv);
-            line[2 * iz] = tv;
+            line[2 * cq_pad(iz)] = tv;
         }
         st4(J.tot + g, o);
     }
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -636,10 +668,10 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, c
     load_tw(tw, J.twx, nxx);
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, TK, tw, nullptr);
-    krow_mult<TK>(J, buf, nullptr, nullptr, k0, 3);   // T = |k|^1.01 total (acsglw.cu:578-588)
+    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
+    krow_mult<TK>(J, buf, k0, 3);   // T = |k|^1.01 total (acsglw.cu:578-588)
     __syncthreads();
-    cq_fft_inv(J.px, buf, nxx, TK, tw);
+    cq_fft_inv(J.px, buf, cq_line_stride(nxx), TK, tw);
     krow_store<TK>(buf, J, J.s1, k0);
 }
 
@@ -653,24 +685,28 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     extern __shared__ float2 lds[];
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
-    float *acc = reinterpret_cast<float *>(buf + L * nzz);
+    float4 *acc = col_acc(lds, nzz, L);
+    float *accf = reinterpret_cast<float *>(acc);   // accf[c * 4 nq + iz]
+    const int AS = 4 * nq;
     load_tw(tw, J.twz, nzz);
     col_insert<CW>(J, buf, x0, J.s1);   // T
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq;
         const float *line = col_line(bufs, nzz, c);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) acc[c * nzz + iz] = line[2 * iz];
+            if (iz < nzz) f4set(a, k, line[2 * cq_pad(iz)]);
         }
+        acc[e] = a;
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     // p += rho v^2 dt lam (total - s dt tao T / 2) - s dt tao aux; source.
     // s = +1 attenuates (acsglw.cu:1500-1537); s = -1 compensates (Q-RTM,
     // sgpstd-qrtm/acsglw.cu:1769-1841)
@@ -683,16 +719,18 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         const float *line = col_line(bufs, nzz, c);
         const size_t g = (size_t)ix * ldz + 4 * zq;
         const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
-                     qe = ld4(J.qe + g);
+                     qe = ld4(J.qe + g), tt = acc[e];
+        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * acc[c * nzz + iz]) -
-                      s * f4get(qe, k) * line[2 * iz];
+            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * f4get(tt, k)) -
+                      s * f4get(qe, k) * line[2 * cq_pad(iz)];
             if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
-            acc[c * nzz + iz] = v;
+            f4set(pn, k, v);
         }
+        acc[e] = pn;
     }
     __syncthreads();
     if (rcv) {
@@ -700,36 +738,33 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         // sgpstd-qrtm/acsglw.cu:1809-1841)
         const int tile = x0 / CW;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT)
-            acc[(J.inj_x[r] - x0) * nzz + J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+            accf[(J.inj_x[r] - x0) * AS + J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         __syncthreads();
     }
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const size_t g = (size_t)ix * ldz + 4 * zq;
-        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) f4set(pn, k, acc[c * nzz + 4 * zq + k]);
+        const float4 pn = acc[e];
         st4(J.p + g, pn);
         if (rcv) interior_accum(J, J.img, ix, 4 * zq, pn, ld4(J.psrc + g));
         else if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, 4 * zq, pn, pn);
     }
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return acc[c * nzz + z]; });
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return accf[c * AS + z]; });
     // p -> LDS (packed columns) for F_z p and dp/dz
     for (int e = threadIdx.x; e < CW * nzz; e += CQ_NT) {
         const int c = e / nzz, iz = e - c * nzz;
-        col_line(bufs, nzz, c)[2 * iz] = (x0 + c < J.nxx) ? acc[c * nzz + iz] : 0.f;
+        col_line(bufs, nzz, c)[2 * cq_pad(iz)] = (x0 + c < J.nxx) ? accf[c * AS + iz] : 0.f;
     }
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, nullptr);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
     __syncthreads();
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, nzz, L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, nzz, L, tw);
+    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
+    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     col_div_phase<CW, true>(J, bufs, x0);
 }
 
@@ -745,11 +780,11 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, c
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.p, ldz, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxp);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
+    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), L, tw, J.mxp);
+    cq_fft_inv(J.px, buf, cq_line_stride(nxx), L, tw);
     row_vx_phase<Q>(J, buf, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, nxx, L, tw, J.mxm);
-    cq_fft_inv(J.px, buf, nxx, L, tw);
+    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), L, tw, J.mxm);
+    cq_fft_inv(J.px, buf, cq_line_stride(nxx), L, tw);
     row_dvx_phase<Q, true>(J, buf, z0);
 }
