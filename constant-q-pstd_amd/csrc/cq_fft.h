@@ -217,17 +217,33 @@ template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
 template <bool INV> struct Dft<17, INV> : DftOdd<17, INV> {};
 
+// Global address space qualifier for device pointers (global_load/store
+// instead of flat accesses).  It only exists in the device compilation pass;
+// the host pass, which never dereferences these pointers, sees plain ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CQ_GLOBAL __attribute__((address_space(1)))
+#else
+#define CQ_GLOBAL
+#endif
+
+// Dynamic LDS of every kernel.  The out-of-line stage functions address it
+// through this symbol (by float2 offsets) so that their accesses compile to
+// LDS instructions rather than generic (flat) ones.
+extern __shared__ float2 cq_lds[];
+
 // ---------------------------------------------------------------------------
 // one Stockham stage over L lines in LDS
 // ---------------------------------------------------------------------------
-// buf  : L lines, stride LS (float2)
-// tw   : LDS table, tw[j] = w_n^j
+// bo   : offset of the first line in cq_lds; L lines LS slots apart
+// to   : offset of the twiddle table, tw[j] = w_n^j
 // Ns   : product of the radices of the previous stages
 // mh   : optional multiplier (global, length n) applied to the outputs
 template <int R, bool INV>
-__device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
-                                         const float2 *tw, const float2 *__restrict__ mh)
+__device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
+                                      const float2 CQ_GLOBAL *__restrict__ mh)
 {
+    float2 *const buf = cq_lds + bo;
+    const float2 *const tw = cq_lds + to;
     constexpr int NB = cq_nb(R);
     const int m = n / R;
     const int total = L * m;
@@ -278,33 +294,35 @@ __device__ __noinline__ void cq_stage(float2 *buf, int LS, int n, int L, int Ns,
 }
 
 template <bool INV>
-__device__ __forceinline__ void cq_stage_rt(int R, float2 *buf, int LS, int n, int L, int Ns,
-                                            const float2 *tw, const float2 *__restrict__ mh)
+__device__ __forceinline__ void cq_stage_rt(int R, int bo, int LS, int n, int L, int Ns, int to,
+                                            const float2 CQ_GLOBAL *__restrict__ mh)
 {
     switch (R) {
-    case 2: cq_stage<2, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 3: cq_stage<3, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 4: cq_stage<4, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 5: cq_stage<5, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 7: cq_stage<7, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 8: cq_stage<8, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 9: cq_stage<9, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 11: cq_stage<11, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 13: cq_stage<13, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 16: cq_stage<16, INV>(buf, LS, n, L, Ns, tw, mh); break;
-    case 17: cq_stage<17, INV>(buf, LS, n, L, Ns, tw, mh); break;
+    case 2: cq_stage<2, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 3: cq_stage<3, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 4: cq_stage<4, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 5: cq_stage<5, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 7: cq_stage<7, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 8: cq_stage<8, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 9: cq_stage<9, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 11: cq_stage<11, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 13: cq_stage<13, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 16: cq_stage<16, INV>(bo, LS, n, L, Ns, to, mh); break;
+    case 17: cq_stage<17, INV>(bo, LS, n, L, Ns, to, mh); break;
     default: break;
     }
 }
 
-// Forward transform of L lines, optional multiplier fused into the last stage.
+// Forward transform of L lines at buf, twiddle table at tw (both inside
+// cq_lds), optional multiplier fused into the last stage.
 __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,
-                                           const float2 *tw, const float2 *__restrict__ mh)
+                                           const float2 *tw, const float2 CQ_GLOBAL *__restrict__ mh)
 {
+    const int bo = (int)(buf - cq_lds), to = (int)(tw - cq_lds);
     int Ns = 1;
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<false>(R, buf, LS, p.n, L, Ns, tw, (s == p.ns - 1) ? mh : nullptr);
+        cq_stage_rt<false>(R, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
         Ns *= R;
     }
 }
@@ -312,10 +330,11 @@ __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS,
 __device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS, int L,
                                            const float2 *tw)
 {
+    const int bo = (int)(buf - cq_lds), to = (int)(tw - cq_lds);
     int Ns = 1;
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<true>(R, buf, LS, p.n, L, Ns, tw, nullptr);
+        cq_stage_rt<true>(R, bo, LS, p.n, L, Ns, to, nullptr);
         Ns *= R;
     }
 }

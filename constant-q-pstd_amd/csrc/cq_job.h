@@ -8,40 +8,61 @@
 // Layout of every real field: x-major, z fastest, leading dimension ldz
 // (nzz rounded up to 32 floats = 128 B, padding rows kept at zero).
 // Half spectra in kz (viscoacoustic path): float2 [nxx][ldk], kz in [0, nzh).
+//
+// The fields of a job record.  G qualifies the device pointers: empty in the
+// host's JobD, the global address space in the kernels' JobG view of the same
+// bytes, so that kernel accesses compile to global rather than flat
+// instructions.
+#define CQ_JOB_FIELDS(G)                                                                          \
+    int kind, nxx, nzz, ldz, pml, nt, xsg, zsg, nr;                                               \
+    int nzh, ldk;                                                                                 \
+    int role;          /* 0: source / modeling wavefield, 1: receiver wavefield (RTM) */          \
+    int nx, nz;        /* interior size (image and illumination) */                               \
+    int gather;        /* record rec[t][i] (modeling) */                                          \
+    float qsign;       /* +1 attenuating (modeling), -1 compensating (Q-RTM) */                   \
+    int stab;          /* stabilised |k|^1.01 multiplier for aux (Q-RTM) */                       \
+    float eta, taoc, dt; /* stabiliser (sgpstd-qrtm/acsglw.cu:1895-1924) */                       \
+    float src_scale;   /* dt * 1000 (acps4.cu:1491) */                                            \
+    float dkx, dkz, invN; /* wavenumber steps (funca.c:555-556) and 1/(nxx nzz) */                \
+    CqPlan px, pz;                                                                                \
+    const float2 G *twx;                                                                          \
+    const float2 G *twz;                                                                          \
+    /* Hermitian-symmetrised 1-D multipliers / n: v->p (m), p->v (p), d2 (2) */                   \
+    const float2 G *mxm; const float2 G *mxp; const float2 G *mx2;                                \
+    const float2 G *mzm; const float2 G *mzp; const float2 G *mz2;                                \
+    /* wavefield state */                                                                         \
+    float G *p; float G *vx; float G *vz; float G *div;                                           \
+    float G *lz; float G *dvx; float G *dvz; float G *tot;                                        \
+    /* per-cell coefficients; acoustic: rho v^2 dt, rho v^4 dt^3/12 */                            \
+    const float G *c1; const float G *c2; const float G *dtbx; const float G *dtbz;               \
+    /* viscoacoustic (see build_job) */                                                           \
+    const float G *qa; const float G *qb; const float G *qe;                                      \
+    const float G *qg; const float G *qy2; const float G *qcc;                                    \
+    /* CPML memory variables, strips only */                                                      \
+    float G *psix; float G *phix; float G *psiz; float G *phiz;                                   \
+    const float G *iax; const float G *ibx; const float G *hax; const float G *hbx;               \
+    const float G *iaz; const float G *ibz; const float G *haz; const float G *hbz;               \
+    /* half-spectrum buffers (viscoacoustic): s2 is state (F_z p), s1/s3/s4 scratch */            \
+    float2 G *s1; float2 G *s2; float2 G *s3; float2 G *s4;                                       \
+    const float G *rik;                                                                           \
+    float G *rec;                                                                                 \
+    /* gather, by x tile */                                                                       \
+    const int G *rcv_x; const int G *rcv_z; const int G *rcv_i; const int G *rcv_off;             \
+    /* RTM: image and illumination over the interior (nx*nz), source p (receiver role), */        \
+    /* nt*nr observed data, injection cells by row or x tile */                                   \
+    float G *img; float G *ssg;                                                                   \
+    const float G *psrc;                                                                          \
+    const float G *obs;                                                                           \
+    const int G *inj_x; const int G *inj_z; const int G *inj_i; const int G *inj_off;
+
 struct JobD {
-    int kind, nxx, nzz, ldz, pml, nt, xsg, zsg, nr;
-    int nzh, ldk;
-    int role;                  // 0: source / modeling wavefield, 1: receiver wavefield (RTM)
-    int nx, nz;                // interior size (image and illumination)
-    int gather;                // record rec[t][i] (modeling)
-    float qsign;               // +1 attenuating (modeling), -1 compensating (Q-RTM)
-    int stab;                  // stabilised |k|^1.01 multiplier for aux (Q-RTM)
-    float eta, taoc, dt;       // stabiliser (sgpstd-qrtm/acsglw.cu:1895-1924)
-    float src_scale;           // dt * 1000 (acps4.cu:1491)
-    float dkx, dkz, invN;      // wavenumber steps (funca.c:555-556) and 1/(nxx nzz)
-    CqPlan px, pz;
-    const float2 *twx, *twz;
-    // Hermitian-symmetrised 1-D multipliers / n: v->p (m), p->v (p), d2 (2)
-    const float2 *mxm, *mxp, *mx2, *mzm, *mzp, *mz2;
-    // wavefield state
-    float *p, *vx, *vz, *div, *lz, *dvx, *dvz, *tot;
-    // per-cell coefficients
-    const float *c1, *c2, *dtbx, *dtbz;                 // acoustic: rho v^2 dt, rho v^4 dt^3/12
-    const float *qa, *qb, *qe, *qg, *qy2, *qcc;         // viscoacoustic (see build_job)
-    // CPML memory variables, strips only
-    float *psix, *phix, *psiz, *phiz;
-    const float *iax, *ibx, *hax, *hbx, *iaz, *ibz, *haz, *hbz;
-    // half-spectrum buffers (viscoacoustic): s2 is state (F_z p), s1/s3/s4 scratch
-    float2 *s1, *s2, *s3, *s4;
-    const float *rik;
-    float *rec;
-    const int *rcv_x, *rcv_z, *rcv_i, *rcv_off;          // gather, by x tile
-    // RTM
-    float *img, *ssg;                                    // nx*nz interior
-    const float *psrc;                                   // source p (receiver role)
-    const float *obs;                                    // nt*nr observed data
-    const int *inj_x, *inj_z, *inj_i, *inj_off;          // injection cells, by row or x tile
+    CQ_JOB_FIELDS()
 };
+
+struct JobG {
+    CQ_JOB_FIELDS(CQ_GLOBAL)
+};
+static_assert(sizeof(JobD) == sizeof(JobG), "JobD and JobG share one layout");
 
 // per-launch flags
 #define CQ_F_SSG 1   // accumulate the illumination ssg += p^2 (first forward sweep)

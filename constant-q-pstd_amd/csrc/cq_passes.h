@@ -12,8 +12,8 @@
 #pragma once
 #include "cq_job.h"
 
-__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+__device__ __forceinline__ float4 ld4(const float CQ_GLOBAL *p) { return *reinterpret_cast<const float4 CQ_GLOBAL *>(p); }
+__device__ __forceinline__ void st4(float CQ_GLOBAL *p, float4 v) { *reinterpret_cast<float4 CQ_GLOBAL *>(p) = v; }
 __device__ __forceinline__ float f4get(const float4 &v, int k)
 {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
@@ -26,7 +26,7 @@ __device__ __forceinline__ void f4set(float4 &v, int k, float s)
     else v.w = s;
 }
 
-__device__ __forceinline__ void load_tw(float2 *tw, const float2 *__restrict__ g, int n)
+__device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__restrict__ g, int n)
 {
     for (int i = threadIdx.x; i < n; i += CQ_NT) tw[i] = g[i];
 }
@@ -35,7 +35,7 @@ __device__ __forceinline__ void load_tw(float2 *tw, const float2 *__restrict__ g
 // row helpers (real rows, packed)
 // ===========================================================================
 template <int Q>
-__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float *__restrict__ src, int ldz, int z0)
+__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int ldz, int z0)
 {
     const int LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
@@ -49,7 +49,7 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float *__re
 // x-CPML on dp/dx (in LDS) and the vx update; leaves vx_new in LDS
 // (vx_cal_kernel, sgpstd-qmodeling/acps4.cu:1274-1303)
 template <int Q>
-__device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
+__device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 {
     const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
@@ -60,7 +60,7 @@ __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
         const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float *php = J.phix + (size_t)sx * ldz + z0 + 4 * q;
+            float CQ_GLOBAL *php = J.phix + (size_t)sx * ldz + z0 + 4 * q;
             const float hb = J.hbx[ix], ha = J.hax[ix];
             float4 ph = ld4(php);
             ph.x = hb * ph.x + ha * dp.x;
@@ -85,7 +85,7 @@ __device__ __forceinline__ void row_vx_phase(const JobD &J, float2 *buf, int z0)
 // (update_vxx_vzz, acps4.cu:1111-1144).  TO_DIV: add dvz (already computed
 // by the column pass) and store div instead (sumpxpz, acps4.cu:1146-1172).
 template <int Q, bool TO_DIV = false>
-__device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, int z0)
+__device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, int z0)
 {
     const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
@@ -95,7 +95,7 @@ __device__ __forceinline__ void row_dvx_phase(const JobD &J, const float2 *buf, 
         float4 dv = make_float4(a.x, a.y, b.x, b.y);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float *psp = J.psix + (size_t)sx * ldz + z0 + 4 * q;
+            float CQ_GLOBAL *psp = J.psix + (size_t)sx * ldz + z0 + 4 * q;
             const float ib = J.ibx[ix], ia = J.iax[ix];
             float4 ps = ld4(psp);
             ps.x = ib * ps.x + ia * dv.x;
@@ -133,7 +133,7 @@ __device__ __forceinline__ float4 *col_acc(float2 *lds, int nzz, int L)
 }
 
 template <int CW>
-__device__ __forceinline__ void col_fill(float *bufs, const JobD &J, const float *__restrict__ src, int x0)
+__device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float CQ_GLOBAL *__restrict__ src, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
@@ -149,7 +149,7 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobD &J, const float
 
 // gather rec[t][i] from a column tile held as plain floats (src(c, iz))
 template <int CW, typename F>
-__device__ __forceinline__ void col_gather(const JobD &J, int x0, int t, F src)
+__device__ __forceinline__ void col_gather(const JobG &J, int x0, int t, F src)
 {
     const int tile = x0 / CW;
     const int r0 = J.rcv_off[tile], r1 = J.rcv_off[tile + 1];
@@ -160,7 +160,7 @@ __device__ __forceinline__ void col_gather(const JobD &J, int x0, int t, F src)
 // z-CPML on dp/dz and the vz update; leaves vz_new in LDS
 // (vz_cal_kernel, acps4.cu:1305-1333)
 template <int CW>
-__device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
+__device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
@@ -177,7 +177,7 @@ __device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
             float dp = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float *php = J.phiz + (size_t)ix * (2 * pml) + sz;
+                float CQ_GLOBAL *php = J.phiz + (size_t)ix * (2 * pml) + sz;
                 const float ph = J.hbz[iz] * (*php) + J.haz[iz] * dp;
                 *php = ph;
                 dp += ph;
@@ -194,7 +194,7 @@ __device__ __forceinline__ void col_vz_phase(const JobD &J, float *bufs, int x0)
 // acps4.cu:1111-1172); stores div and leaves it in LDS.  TO_DVZ: store dvz
 // alone (the row pass adds dvx).
 template <int CW, bool TO_DVZ = false>
-__device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0)
+__device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
             float d = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
+                float CQ_GLOBAL *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
                 const float ps = J.ibz[iz] * (*psp) + J.iaz[iz] * d;
                 *psp = ps;
                 d += ps;
@@ -234,8 +234,8 @@ __device__ __forceinline__ void col_div_phase(const JobD &J, float *bufs, int x0
 // k in [0, nzh), and store them.  With mh != null also multiply Z by mh in
 // place (each thread owns the pair {k, -k}).
 template <int CW>
-__device__ __forceinline__ void col_extract(const JobD &J, float2 *buf, int x0, float2 *__restrict__ S,
-                                            const float2 *__restrict__ mh)
+__device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, float2 CQ_GLOBAL *__restrict__ S,
+                                            const float2 CQ_GLOBAL *__restrict__ mh)
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
@@ -260,7 +260,7 @@ __device__ __forceinline__ void col_extract(const JobD &J, float2 *buf, int x0, 
 // two Hermitian half spectra (imaginary parts dropped at the self-conjugate
 // bins k = 0 and k = nzz/2, as a complex-to-real transform does).
 template <int CW>
-__device__ __forceinline__ void col_insert(const JobD &J, float2 *buf, int x0, const float2 *__restrict__ S)
+__device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, const float2 CQ_GLOBAL *__restrict__ S)
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
@@ -285,27 +285,27 @@ __device__ __forceinline__ void col_insert(const JobD &J, float2 *buf, int x0, c
 // kz-row helpers (half spectra, TK complex lines of length nxx)
 // ===========================================================================
 template <int TK>
-__device__ __forceinline__ void krow_fill(float2 *buf, const JobD &J, const float2 *__restrict__ S, int k0)
+__device__ __forceinline__ void krow_fill(float2 *buf, const JobG &J, const float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
     constexpr int H = TK / 2;
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
-        const float4 v = *reinterpret_cast<const float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
+        const float4 v = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
         buf[(2 * h) * LS + cq_pad(ix)] = make_float2(v.x, v.y);
         buf[(2 * h + 1) * LS + cq_pad(ix)] = make_float2(v.z, v.w);
     }
 }
 
 template <int TK>
-__device__ __forceinline__ void krow_store(const float2 *buf, const JobD &J, float2 *__restrict__ S, int k0)
+__device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
     constexpr int H = TK / 2;
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
         const float2 a = buf[(2 * h) * LS + cq_pad(ix)], b = buf[(2 * h + 1) * LS + cq_pad(ix)];
-        *reinterpret_cast<float4 *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
+        *reinterpret_cast<float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
     }
 }
 
@@ -328,7 +328,7 @@ __device__ __forceinline__ float frac_symbol(int which, float k2)
 // line l x |k|^(1/16) / N, line TK + l x |k|^2 / N and line 2 TK + l x
 // |k|^(33/16) / N (the frac_symbol values 0, 1, 2).
 template <int TK>
-__device__ __forceinline__ void krow_mult3(const JobD &J, float2 *buf, int k0)
+__device__ __forceinline__ void krow_mult3(const JobG &J, float2 *buf, int k0)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
@@ -352,7 +352,7 @@ __device__ __forceinline__ void krow_mult3(const JobD &J, float2 *buf, int k0)
 // e = eta * 1e-6, tc = max tao, tt = t_aux dt (multt_kernel,
 // sgpstd-qrtm/acsglw.cu:1895-1924).
 template <int TK>
-__device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, int k0, int which, float tt = 0.f)
+__device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, int which, float tt = 0.f)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
@@ -385,7 +385,7 @@ __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, int k0, in
 #define CQ_ITEM_PROLOGUE                      \
     const int2 it = items[blockIdx.x];        \
     if (it.x < 0) return;                     \
-    const JobD &J = jobs[it.x];               \
+    const JobG &J = reinterpret_cast<const JobG *>(jobs)[it.x]; \
     const int t = step;                       \
     if (t >= J.nt) return;                    \
     (void)t_aux;                              \
@@ -394,7 +394,7 @@ __device__ __forceinline__ void krow_mult(const JobD &J, float2 *buf, int k0, in
 // dst[interior(ix, iz0 + k)] += a.k * b.k for the 4 depths of a float4
 // (image: rtm_image, sgpstd-qrtm/acps4.cu:1694-1721; illumination: addsa_kernel,
 // :1656)
-__device__ __forceinline__ void interior_accum(const JobD &J, float *dst, int ix, int iz0, float4 a, float4 b)
+__device__ __forceinline__ void interior_accum(const JobG &J, float CQ_GLOBAL *dst, int ix, int iz0, float4 a, float4 b)
 {
     const int xi = ix - J.pml;
     if (xi < 0 || xi >= J.nx) return;
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, ldz = J.ldz, z0 = it.y, LS = cq_line_stride(nxx);
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.div, ldz, z0);
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
     load_tw(tw, J.twz, nzz);
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
     load_tw(tw, J.twz, nzz);
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y, LS = cq_line_stride(nxx);
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;   // 3 TK lines: A, K, B
     load_tw(tw, J.twx, nxx);
     krow_fill<TK>(buf, J, J.s1, k0);
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
     float4 *acc = col_acc(lds, nzz, L);
@@ -663,7 +663,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, c
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
     krow_fill<TK>(buf, J, J.s1, k0);
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
     float4 *acc = col_acc(lds, nzz, L);
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, c
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, ldz = J.ldz, z0 = it.y;
-    extern __shared__ float2 lds[];
+    float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.p, ldz, z0);
