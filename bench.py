@@ -86,6 +86,69 @@ def cpu_baseline(job, steps, threads):
     return j["nxx"] * j["nzz"] * steps / dt / 1e6, dt
 
 
+def serve_bench(rate, window_s, seed, chunk):
+    """Serving workload (Poisson arrivals of mixed acoustic / viscoacoustic /
+    Q-RTM requests, each with its own window, nt, dx and wavelet) through the
+    engine's continuous-batching server.  Requests are generated before the
+    clock starts and submitted at their arrival times; the server admits them
+    into the running launches.  Latency = completion - arrival, wall clock."""
+    from cqpy import engine, host, synth
+    reqs = synth.serve_jobs(rate=rate, window_s=window_s, seed=seed)
+    mx = max(j["nxx"] for _, j, _ in reqs)
+    mz = max(j["nzz"] for _, j, _ in reqs)
+    useful = executed = 0
+    for _, j, kind in reqs:
+        c = j["nxx"] * j["nzz"]
+        if kind == 3:
+            useful += c * 2 * j["nt"]
+            executed += c * (host.numforw(j["nt"], j["snaps"]) + j["nt"])
+        else:
+            useful += c * j["nt"]
+            executed += c * j["nt"]
+    ctx = engine.Context(0)
+    srv = ctx.server(mx, mz)
+    pending = list(reqs)
+    info, lat = {}, {}
+    t0 = time.perf_counter()
+    while pending or len(lat) < len(info):
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            arr, job, kind = pending.pop(0)
+            info[srv.submit(job)] = (arr, kind)
+        running, queued = srv.advance(chunk)
+        now = time.perf_counter() - t0
+        for i, (arr, kind) in info.items():
+            if i not in lat and srv.state(i) == engine.DONE:
+                lat[i] = now - arr
+        if running == 0 and queued == 0 and pending:
+            time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    srv.close()
+    ctx.close()
+    first = reqs[0][0]
+
+    def pct(v, p):
+        v = sorted(v)
+        return round(v[min(len(v) - 1, int(round(p / 100.0 * (len(v) - 1))))], 3)
+
+    names = {0: "acoustic", 1: "viscoacoustic", 3: "qrtm"}
+    per = {}
+    for k, nm in names.items():
+        v = [lat[i] for i, (a, kk) in info.items() if kk == k]
+        if v:
+            per[nm] = {"jobs": len(v), "p50_s": pct(v, 50), "p99_s": pct(v, 99)}
+    allv = list(lat.values())
+    return {"workload": "serve: Poisson %.0f jobs/s for %.1f s (%d jobs; 50%% acoustic, 30%% "
+                        "viscoacoustic, 20%% Q-RTM with 40 checkpoints), windows %d-%d x %d-%d "
+                        "cells + 40-pt CPML, nt 2000-8000, dx 5-12.5 m, Ricker 10-30 Hz"
+                        % (rate, window_s, len(reqs), 512, 1216, 256, 600),
+            "jobs": len(reqs), "wall_s": round(wall, 3),
+            "useful_mcell_steps_per_s": round(useful / (wall - first) / 1e6, 1),
+            "executed_mcell_steps_per_s": round(executed / (wall - first) / 1e6, 1),
+            "p50_s": pct(allv, 50), "p99_s": pct(allv, 99), "per_class": per,
+            "steps_per_advance": chunk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +163,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a streaming copy of this many GB (counter calibration)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) phase")
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-window", type=float, default=1.0)
+    ap.add_argument("--serve-chunk", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one "
@@ -177,6 +244,12 @@ def main():
         plain = pj["nxx"] * pj["nzz"] * args.plain_steps / pel / 1e6
     ctx.close()
 
+    serve = None
+    if not args.no_serve:
+        print("serving phase", file=sys.stderr, flush=True)
+        serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk)
+        print("serve:", json.dumps(serve), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -210,6 +283,7 @@ def main():
                      "step_bytes": step_bytes,
                      "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out))
 

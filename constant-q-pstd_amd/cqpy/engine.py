@@ -5,6 +5,7 @@ is missing the calls raise instead of falling back to anything else.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -31,11 +32,20 @@ class JobDesc(C.Structure):
     ]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("max_nxx", C.c_int), ("max_nzz", C.c_int), ("hbm_budget", C.c_longlong),
+                ("max_jobs", C.c_int)]
+
+
+QUEUED, RUNNING, DONE = 0, 1, 2
+
 SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_create",
            "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",
            "cq_batch_read_p", "cq_batch_time", "cq_batch_bytes", "cq_batch_destroy",
            "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times",
-           "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image"]
+           "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image",
+           "cq_server_open", "cq_server_submit", "cq_server_advance", "cq_server_poll",
+           "cq_server_job_steps", "cq_server_result", "cq_server_release", "cq_server_close"]
 
 _lib = None
 
@@ -103,9 +113,13 @@ def make_desc(job, keep):
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self.children = weakref.WeakSet()   # open batches and servers
         _chk(lib().cq_open(C.c_int(device), C.byref(self.h)))
 
     def close(self):
+        """Closes the batches and servers still open, then the context."""
+        for c in list(self.children):
+            c.close()
         if self.h:
             lib().cq_close(self.h)
             self.h = C.c_void_p()
@@ -126,6 +140,9 @@ class Context:
     def batch(self, jobs):
         return Batch(self, jobs)
 
+    def server(self, max_nxx, max_nzz, hbm_budget=0, max_jobs=0):
+        return Server(self, max_nxx, max_nzz, hbm_budget, max_jobs)
+
     def copy_calibrate(self, nbytes, reps=10):
         """ms per streaming copy of nbytes (reads nbytes, writes nbytes)."""
         ms = C.c_double()
@@ -143,6 +160,7 @@ class Batch:
             arr[i] = make_desc(j, keep)
         self.h = C.c_void_p()
         _chk(lib().cq_batch_create(ctx.h, arr, C.c_int(len(jobs)), C.byref(self.h)))
+        ctx.children.add(self)
 
     def step(self, n):
         _chk(lib().cq_batch_step(self.h, C.c_int(n)))
@@ -205,6 +223,70 @@ class Batch:
     def close(self):
         if self.h:
             lib().cq_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server (cq_server_* of include/cq.h)."""
+
+    def __init__(self, ctx, max_nxx, max_nzz, hbm_budget=0, max_jobs=0):
+        self.ctx = ctx
+        self.jobs = {}
+        d = ServerDesc(int(max_nxx), int(max_nzz), int(hbm_budget), int(max_jobs))
+        self.h = C.c_void_p()
+        _chk(lib().cq_server_open(ctx.h, C.byref(d), C.byref(self.h)))
+        ctx.children.add(self)
+
+    def submit(self, job):
+        keep = []
+        d = make_desc(job, keep)
+        i = C.c_int()
+        _chk(lib().cq_server_submit(self.h, C.byref(d), C.byref(i)))
+        self.jobs[i.value] = job
+        return i.value
+
+    def advance(self, nsteps):
+        """Runs up to nsteps global steps; returns (running, queued)."""
+        r, q = C.c_int(), C.c_int()
+        _chk(lib().cq_server_advance(self.h, C.c_int(int(nsteps)), C.byref(r), C.byref(q)))
+        return r.value, q.value
+
+    def state(self, i):
+        s = C.c_int()
+        _chk(lib().cq_server_poll(self.h, C.c_int(i), C.byref(s)))
+        return s.value
+
+    def steps(self, i):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _chk(lib().cq_server_job_steps(self.h, C.c_int(i), C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def result(self, i):
+        """Gather (nt, nr) of a modeling job, (img, ssg) of an RTM job."""
+        j = self.jobs[i]
+        if int(j["kind"]) >= 2:
+            n = (int(j["nxx"]) - 2 * int(j["pml"])) * (int(j["nzz"]) - 2 * int(j["pml"]))
+            img, ssg = np.zeros(n, np.float32), np.zeros(n, np.float32)
+            _chk(lib().cq_server_result(self.h, C.c_int(i), img.ctypes.data_as(_F), ssg.ctypes.data_as(_F)))
+            return img, ssg
+        nr = len(j["gxg"])
+        out = np.zeros(int(j["nt"]) * nr, np.float32)
+        _chk(lib().cq_server_result(self.h, C.c_int(i), out.ctypes.data_as(_F), None))
+        return out.reshape(int(j["nt"]), nr)
+
+    def release(self, i):
+        _chk(lib().cq_server_release(self.h, C.c_int(i)))
+        self.jobs.pop(i, None)
+
+    def close(self):
+        if self.h:
+            lib().cq_server_close(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -223,3 +223,46 @@ def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None):
                                     2, np.arange(left, right + 1), 2, pml, qt=q, fo=20.0,
                                     want_k2=want_k2))
     return jobs
+
+
+SERVE_CLASSES = ((0, 0.5), (1, 0.3), (3, 0.2))   # acoustic, viscoacoustic, Q-RTM
+
+
+def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 1216), nz_range=(256, 600),
+               nt_range=(2000, 8000), dx_range=(5.0, 12.5), fd_range=(10.0, 30.0), pml=40, snaps=40):
+    """Serving workload: Poisson arrivals (mean `rate` jobs/s) over `window_s`
+    seconds of mixed requests, 50% acoustic forward, 30% viscoacoustic forward,
+    20% Q-RTM single shot.  Each request draws its window (nx columns, nz
+    rows, widened to FFT-friendly padded sizes), nt, dx = dz, Ricker peak
+    frequency and a layered vp/rho/Q model (vp 1500-4500 m/s, Q 30-300) from
+    its own seed; dt = 0.4 dx / vmax.  The source sits at the window centre,
+    receivers on every surface column.  Q-RTM requests carry seeded random
+    observed data (the timing does not depend on its content) and `snaps`
+    checkpoints.  Returns [(arrival seconds, job dict, class)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = rng.exponential(1.0 / rate)
+    k = 0
+    while t < window_s:
+        r = np.random.default_rng(seed * 1000 + k)
+        u = r.uniform()
+        kind = SERVE_CLASSES[0][0] if u < 0.5 else (SERVE_CLASSES[1][0] if u < 0.8 else SERVE_CLASSES[2][0])
+        nx = fft_friendly(int(r.integers(nx_range[0], nx_range[1] + 1)) + 2 * pml) - 2 * pml
+        nz = fft_friendly(int(r.integers(nz_range[0], nz_range[1] + 1)) + 2 * pml) - 2 * pml
+        nt = int(r.integers(nt_range[0], nt_range[1] + 1))
+        dx = float(np.round(r.uniform(*dx_range) * 4) / 4)
+        fd = float(np.round(r.uniform(*fd_range)))
+        vmax = 4500.0
+        dt = float(np.float32(0.4 * dx / vmax))
+        vp, rho, q = layered_model(nx, nz, dx, seed=seed * 1000 + k, vmin=1500.0, vmax=vmax,
+                                   qrange=(30.0, 300.0))
+        job = make_window_job(kind, vp, rho, nx, nz, dx, dx, dt, nt, fd, 0, nx, nx // 2, 2,
+                              np.arange(nx), 2, pml, qt=q, fo=fd)
+        if kind == 3:
+            job["obs"] = r.standard_normal((nt, len(job["gxg"]))).astype(np.float32)
+            job["snaps"] = snaps
+            job["etas"] = job["etar"] = 1e-20
+        out.append((float(t), job, kind))
+        t += rng.exponential(1.0 / rate)
+        k += 1
+    return out

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,9 @@ struct cq_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     std::map<int, float2 *> twiddles;  // n -> device table w_n^j, j < n
+    // open batches and servers; cq_close destroys what is left of them
+    std::set<cq_batch_s *> batches;
+    std::set<cq_server_s *> servers;
     ~cq_ctx_s()
     {
         for (auto &kv : twiddles) (void)hipFree(kv.second);
@@ -227,6 +231,52 @@ static void bind_state(JobD &d, char *base, const StateLayout &s, bool q)
     d.phiz = (float *)(base + s.phiz);
 }
 
+// A checkpoint copy of an RTM shot, issued before its local step `at`:
+// dir 1 state -> slot (takeshot), 0 slot -> state (restore); src0 adds the
+// wavelet's first sample at the source (taken at capo 0).
+struct CkptOp {
+    int at, dir, slot, src0;
+};
+
+// The revolve schedule of an RTM shot of nt steps with `snaps` checkpoints
+// (sgpstd-qrtm/acps4.cu:447-905, sgpstd-qrtm/acsglw.cu:470-1040) as one step
+// record {role, t, t_aux, flags} per executed step plus the checkpoint
+// copies between steps:
+//   advance  : source steps t = ts .. capo, t_aux = t - 1, ssg during the first sweep
+//   firsturn : receiver step t = nt - 1, t_aux = nt
+//   youturn  : receiver step t = capo, t_aux = capo + 1
+//   takeshot / restore : copies; a restore resumes the source at capo + 1
+static int rtm_schedule(int nt, int snaps, std::vector<int4> &steps, std::vector<CkptOp> &ops)
+{
+    cqh_revolve rv;
+    cqh_revolve_init(&rv, nt, snaps);
+    int ts = 0, restart = 0;
+    steps.clear();
+    ops.clear();
+    for (;;) {
+        const int a = cqh_revolve_next(&rv);
+        const int at = (int)steps.size();
+        if (a == CQH_RV_TAKESHOT) {
+            ops.push_back({at, 1, rv.check, rv.capo == 0 ? 1 : 0});
+        } else if (a == CQH_RV_ADVANCE) {
+            for (int t = ts; t <= rv.capo; t++) steps.push_back(make_int4(0, t, t - 1, restart ? 0 : CQ_F_SSG));
+            ts = rv.capo + 1;
+        } else if (a == CQH_RV_FIRSTURN) {
+            restart = 1;
+            steps.push_back(make_int4(1, nt - 1, nt, 0));
+        } else if (a == CQH_RV_YOUTURN) {
+            steps.push_back(make_int4(1, rv.capo, rv.capo + 1, 0));
+        } else if (a == CQH_RV_RESTORE) {
+            ops.push_back({at, 0, rv.check, 0});
+            ts = rv.capo + 1;
+        } else if (a == CQH_RV_TERMINATE) {
+            return CQ_OK;
+        } else {
+            return set_err(CQ_EINVAL, "revolve error, info %d", rv.info);
+        }
+    }
+}
+
 struct JobH {
     JobD d[2];              // role 0: source / modeling wavefield; role 1: receiver wavefield (RTM)
     int nroles = 1;
@@ -239,6 +289,9 @@ struct JobH {
     int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0, kind = 0, nx = 0, nz = 0;
     size_t src_cell = 0;    // xsg * ldz + zsg
     float src0 = 0.f;       // rik[0] * dt * 1000 (checkpoint-0 quirk)
+    int t0 = 0, nsched = 0; // global steps t0 .. t0 + nsched - 1
+    std::vector<CkptOp> ops;  // RTM checkpoint copies, by local step
+    size_t op_next = 0;
 };
 
 // One launch of the per-step sequence.
@@ -246,18 +299,23 @@ enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_QC2, P_QR2, P_QC3, P_QR3, P_COUNT 
 static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1",
                                          "k_q_c2",   "k_q_r2",   "k_q_c3", "k_q_r3"};
 
+// A set of shots advanced together: every launch of a pass covers the tiles
+// of every wavefield in the set, and each wavefield decides from its step
+// record whether it advances at the launch's global step.
 struct cq_batch_s {
     cq_ctx ctx = nullptr;
     std::vector<JobH> jobs;
     bool rtm = false;
-    JobD *djobs = nullptr;                 // [role][job]
-    int2 *ditems[2][P_COUNT] = {};
-    int nitems[2][P_COUNT] = {};
+    // job table: entry i = role 0 of jobs[i], entry cap + i = role 1
+    JobD *djobs = nullptr;
+    size_t djobs_cap = 0;                  // entries per role the table holds
+    int2 *ditems[P_COUNT] = {};            // (table entry, tile origin), XCD-ordered
+    size_t ditems_cap[P_COUNT] = {};
+    int nitems[P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
-    int t = 0, tmax = 0;
-    // RTM schedule state
-    int snaps = 0, rtm_done = 0, rv_info = 0;
+    int maxnxx = 0, maxnzz = 0;
+    int t = 0, tmax = 0;                   // next global step; end of the last wavefield
     bool timing = false;
     std::vector<hipEvent_t> ev;            // start of step, then one per launch
     std::vector<int> ev_pass;
@@ -265,8 +323,7 @@ struct cq_batch_s {
     {
         for (auto &j : jobs) (void)hipFree(j.mem);
         (void)hipFree(djobs);
-        for (int r = 0; r < 2; r++)
-            for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
+        for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[i]);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
@@ -293,11 +350,14 @@ extern "C" int cq_open(int device, cq_ctx *out)
     return CQ_OK;
 }
 
+static void destroy_children(cq_ctx ctx);
+
 extern "C" int cq_close(cq_ctx ctx)
 {
     if (!ctx) return CQ_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    destroy_children(ctx);
     delete ctx;
     return CQ_OK;
 }
@@ -348,13 +408,15 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
     }
 }
 
-static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
+// Allocates and uploads one shot for the batch's tile shapes.  With
+// bytes_only != null only its device footprint is computed.
+static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_only = nullptr)
 {
     cq_ctx ctx = b->ctx;
     const int cw = is_q(j.kind) ? b->cwq : b->cw, tk = b->tk, tz = b->tz;
     const bool q = is_q(j.kind), rtm = is_rtm(j.kind);
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
-    const int ldz = (nzz + 31) & ~31;
+    const int ldz = ((nzz + tz - 1) / tz * tz + 31) & ~31;   // whole row tiles, 128-B rows
     const int nzh = nzz / 2 + 1;
     const int ldk = (nzh + tk - 1) / tk * tk;
     const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
@@ -390,10 +452,11 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
     const size_t o_ckpt = A.take<char>(sl.bytes * (size_t)snaps);
     const size_t o_bx = A.take<float>(N), o_bz = A.take<float>(N);
     const size_t o_f2 = A.take<float>(N), o_f3 = A.take<float>(N);
-    size_t o_q[4] = {0, 0, 0, 0}, o_tot = 0, o_s1 = 0, o_s3 = 0, o_s4 = 0;
+    size_t o_q[4] = {0, 0, 0, 0}, o_tot = 0, o_scr = 0, o_s1 = 0, o_s3 = 0, o_s4 = 0;
     if (q) {
         for (int i = 0; i < 4; i++) o_q[i] = A.take<float>(N);
         o_tot = A.take<float>(N);
+        o_scr = A.take<float>(N);
         o_s1 = A.take<float2>((size_t)nxx * ldk);
         o_s3 = A.take<float2>((size_t)nxx * ldk);
         o_s4 = A.take<float2>((size_t)nxx * ldk);
@@ -413,7 +476,21 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
         o_inj = A.take<int>((size_t)3 * std::max(j.nr, 1));
         o_ioff = A.take<int>((size_t)ntile_inj + 1);
     }
+    std::vector<int4> hsched;
+    size_t o_sched = 0;
+    if (rtm) {
+        int rc0 = rtm_schedule(j.nt, j.snaps, hsched, h.ops);
+        if (rc0) return rc0;
+        o_sched = A.take<int4>(hsched.size());
+    }
+    h.nsched = rtm ? (int)hsched.size() : j.nt;
+    h.t0 = 0;
+    h.op_next = 0;
     const size_t total = A.take<char>(0);
+    if (bytes_only) {
+        *bytes_only = total;
+        return CQ_OK;
+    }
     char *mem = nullptr;
     hipError_t e = hipMalloc(&mem, total);
     if (e != hipSuccess) return set_err(CQ_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e));
@@ -437,7 +514,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
         base.qe = (float *)(mem + o_q[0]); base.qg = (float *)(mem + o_q[1]);
         base.qy2 = (float *)(mem + o_q[2]); base.qcc = (float *)(mem + o_q[3]);
         base.tot = (float *)(mem + o_tot);
-        base.s1 = (float2 *)(mem + o_s1); base.s3 = (float2 *)(mem + o_s3); base.s4 = (float2 *)(mem + o_s4);
+        base.scr = (float *)(mem + o_scr);
+        base.s1 =(float2 *)(mem + o_s1); base.s3 = (float2 *)(mem + o_s3); base.s4 = (float2 *)(mem + o_s4);
     }
     float *x1 = (float *)(mem + o_1x), *z1 = (float *)(mem + o_1z);
     base.iax = x1; base.ibx = x1 + nxx; base.hax = x1 + 2 * nxx; base.hbx = x1 + 3 * nxx;
@@ -465,6 +543,9 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
             base.taoc = mx_;
         }
     }
+    base.t0 = 0;
+    base.nsched = h.nsched;
+    base.sched = rtm ? (const int4 *)(mem + o_sched) : nullptr;
     for (int r = 0; r < nrole; r++) {
         h.d[r] = base;
         h.d[r].role = r;
@@ -572,6 +653,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h)
             HIPCHK(hipMemcpyAsync((void *)base.obs, j.obs, 4 * (size_t)j.nt * j.nr, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync((void *)base.inj_x, hinj.data(), 4 * hinj.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync((void *)base.inj_off, hioff.data(), 4 * hioff.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync((void *)base.sched, hsched.data(), sizeof(int4) * hsched.size(), hipMemcpyHostToDevice, s));
     }
     // the host vectors above must outlive the copies
     HIPCHK(hipStreamSynchronize(s));
@@ -608,12 +690,140 @@ static const size_t kLdsTile = CQ_LDS_TILE;
 // LDS bytes of a workgroup holding the twiddle table of an n-point transform
 // and `lines` padded lines of it.
 static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_stride(n)) * 8; }
-// ... plus the 16-byte aligned per-column float4 accumulator of a cw-column
-// tile (col_acc).
-static size_t lds_col_acc(int n, int cw)
+// Candidate tile widths, largest first: z rows per row tile (two per packed
+// line), columns per column tile (two per packed line), kz rows per kz-row
+// tile.  Widths that are not powers of two fill a workgroup's 512 threads
+// better when a power of two does not fit the LDS budget.
+static const int kTzCand[] = {16, 12, 8};
+static const int kCwCand[] = {32, 24, 16, 12, 8, 4, 2};
+static const int kTkCand[] = {8, 4, 2};
+
+// LDS of each pass for windows up to ax x az (acoustic) and qx x qz
+// (viscoacoustic).
+static void set_lds(cq_batch b, int ax, int az, int qx, int qz)
 {
-    const int off = n + (cw / 2) * cq_line_stride(n);
-    return (size_t)((off + 1) & ~1) * 8 + (size_t)cw * ((n + 3) / 4) * 16;
+    b->lds[P_AROW] = lds_lines(ax, b->tz / 2);
+    b->lds[P_ACOL] = lds_lines(az, b->cw / 2);
+    b->lds[P_QC1] = lds_lines(qz, b->cwq / 2);
+    b->lds[P_QR1] = lds_lines(qx, 3 * b->tk);
+    b->lds[P_QC2] = lds_lines(qz, b->cwq / 2);
+    b->lds[P_QR2] = lds_lines(qx, b->tk);
+    b->lds[P_QC3] = lds_lines(qz, b->cwq / 2);
+    b->lds[P_QR3] = lds_lines(qx, b->tz / 2);
+}
+
+// Tile shapes for the windows a batch will hold: the largest tiles whose
+// transforms fit one workgroup's per-stage capacity (a plan exists) and the
+// LDS budget kLdsTile, else kLdsMax.  Row tiles are shared by both kinds,
+// column tiles are chosen per kind (cw acoustic, cwq viscoacoustic).
+struct TileReq {
+    int nxx, nzz;
+    bool q;
+};
+
+static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
+{
+    int iz = 0, ic = 0, icq = 0, ik = 0;   // indices into the candidate lists
+    int ax = 0, az = 0, qx = 0, qz = 0;
+    for (size_t i = 0; i < req.size(); i++) {
+        const TileReq &j = req[i];
+        if (!has_plan(j.nxx, 1))
+            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", (int)i, j.nxx);
+        if (!has_plan(j.nzz, 1))
+            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", (int)i, j.nzz);
+        const bool q = j.q;
+        auto row_bad = [&](int t, size_t lim) {
+            return !has_plan(j.nxx, (t / 2) * j.nxx) || lds_lines(j.nxx, t / 2) > lim;
+        };
+        auto col_bad = [&](int c, size_t lim) {
+            return !has_plan(j.nzz, (c / 2) * j.nzz) || lds_lines(j.nzz, c / 2) > lim;
+        };
+        auto krow_bad = [&](int k, size_t lim) {
+            return !has_plan(j.nxx, 3 * k * j.nxx) || lds_lines(j.nxx, 3 * k) > lim;
+        };
+        int &ick = q ? icq : ic;
+        while (iz < 2 && row_bad(kTzCand[iz], kLdsTile)) iz++;
+        while (ick < 6 && col_bad(kCwCand[ick], kLdsTile)) ick++;
+        if (q)
+            while (ik < 2 && krow_bad(kTkCand[ik], kLdsTile)) ik++;
+        if (row_bad(kTzCand[iz], kLdsMax) || col_bad(kCwCand[ick], kLdsMax) || (q && krow_bad(kTkCand[ik], kLdsMax)))
+            return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", (int)i, j.nxx, j.nzz);
+        int &mx = q ? qx : ax, &mz = q ? qz : az;
+        mx = std::max(mx, j.nxx);
+        mz = std::max(mz, j.nzz);
+    }
+    b->tz = kTzCand[iz];
+    b->cw = kCwCand[ic];
+    b->cwq = kCwCand[icq];
+    b->tk = kTkCand[ik];
+    b->maxnxx = std::max(ax, qx);
+    b->maxnzz = std::max(az, qz);
+    set_lds(b, ax, az, qx, qz);
+    return CQ_OK;
+}
+
+// Device job table and per-pass work items for the batch's current jobs.
+// Table entry i is role 0 of jobs[i], entry djobs_cap + i its role 1; items
+// cover every tile of every wavefield.  Nothing may be in flight on the
+// stream (the old table and items are overwritten).
+static int rebuild_tables(cq_batch b)
+{
+    hipStream_t s = b->ctx->stream;
+    const size_t n = b->jobs.size();
+    if (n > b->djobs_cap || !b->djobs) {
+        (void)hipFree(b->djobs);
+        b->djobs = nullptr;
+        b->djobs_cap = std::max(n, (size_t)2 * b->djobs_cap);
+        if (b->djobs_cap == 0) b->djobs_cap = 1;
+        HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * 2 * b->djobs_cap));
+    }
+    const int cap = (int)b->djobs_cap;
+    std::vector<JobD> hd(2 * (size_t)cap);
+    memset(hd.data(), 0, sizeof(JobD) * hd.size());
+    std::vector<int2> items[P_COUNT];
+    b->tmax = 0;
+    for (size_t i = 0; i < n; i++) {
+        const JobH &h = b->jobs[i];
+        hd[i] = h.d[0];
+        hd[cap + i] = h.nroles > 1 ? h.d[1] : h.d[0];
+        b->tmax = std::max(b->tmax, h.t0 + h.nsched);
+        for (int r = 0; r < h.nroles; r++) {
+            const int e = r ? cap + (int)i : (int)i;
+            if (!is_q(h.kind)) {
+                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) items[P_AROW].push_back(make_int2(e, z0));
+                for (int x0 = 0; x0 < h.nxx; x0 += b->cw) items[P_ACOL].push_back(make_int2(e, x0));
+            } else {
+                for (int x0 = 0; x0 < h.nxx; x0 += b->cwq) {
+                    items[P_QC1].push_back(make_int2(e, x0));
+                    items[P_QC2].push_back(make_int2(e, x0));
+                    items[P_QC3].push_back(make_int2(e, x0));
+                }
+                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) {
+                    items[P_QR1].push_back(make_int2(e, k0));
+                    items[P_QR2].push_back(make_int2(e, k0));
+                }
+                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) items[P_QR3].push_back(make_int2(e, z0));
+            }
+        }
+    }
+    std::vector<int2> ord[P_COUNT];
+    HIPCHK(hipMemcpyAsync(b->djobs, hd.data(), sizeof(JobD) * hd.size(), hipMemcpyHostToDevice, s));
+    for (int p = 0; p < P_COUNT; p++) {
+        b->nitems[p] = 0;
+        if (items[p].empty()) continue;
+        ord[p] = xcd_order(items[p]);
+        if (ord[p].size() > b->ditems_cap[p]) {
+            (void)hipFree(b->ditems[p]);
+            b->ditems[p] = nullptr;
+            b->ditems_cap[p] = std::max(ord[p].size(), 2 * b->ditems_cap[p]);
+            HIPCHK(hipMalloc(&b->ditems[p], sizeof(int2) * b->ditems_cap[p]));
+        }
+        HIPCHK(hipMemcpyAsync(b->ditems[p], ord[p].data(), sizeof(int2) * ord[p].size(), hipMemcpyHostToDevice, s));
+        b->nitems[p] = (int)ord[p].size();
+    }
+    // the host vectors must outlive the copies
+    HIPCHK(hipStreamSynchronize(s));
+    return CQ_OK;
 }
 
 extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out)
@@ -626,126 +836,43 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     }
     std::unique_ptr<cq_batch_s> b(new cq_batch_s());
     b->ctx = ctx;
-    b->rtm = is_rtm(jobs[0].kind);
-    for (int i = 1; i < njobs; i++) {
-        if (is_rtm(jobs[i].kind) != b->rtm)
-            return set_err(CQ_EINVAL, "a batch holds either modeling or RTM shots (job %d differs)", i);
-        if (b->rtm && (jobs[i].nt != jobs[0].nt || jobs[i].snaps != jobs[0].snaps))
-            return set_err(CQ_EINVAL, "RTM shots of one batch share nt and snaps (job %d differs)", i);
-    }
-    // Tile shapes: the largest tiles whose lines fit one workgroup's per-stage
-    // capacity and the LDS for every shot of the batch.
-    int tz = 16, cw = 32, cwq = 32, tk = 8;
-    int maxnxx = 0, maxnzz = 0;
-    for (int i = 0; i < njobs; i++) {
-        const cq_job_desc &j = jobs[i];
-        if (!has_plan(j.nxx, 1))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nxx);
-        if (!has_plan(j.nzz, 1))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", i, j.nzz);
-        const bool q = is_q(j.kind);
-        auto row_bad = [&](int t, size_t lim) {
-            return !has_plan(j.nxx, (t / 2) * j.nxx) || lds_lines(j.nxx, t / 2) > lim;
-        };
-        auto col_bad = [&](int c, size_t lim) {
-            return !has_plan(j.nzz, (c / 2) * j.nzz) || (q ? lds_col_acc(j.nzz, c) : lds_lines(j.nzz, c / 2)) > lim;
-        };
-        auto krow_bad = [&](int k, size_t lim) {
-            return !has_plan(j.nxx, 3 * k * j.nxx) || lds_lines(j.nxx, 3 * k) > lim;
-        };
-        int &cwk = q ? cwq : cw;
-        while (tz > 8 && row_bad(tz, kLdsTile)) tz /= 2;
-        while (cwk > 2 && col_bad(cwk, kLdsTile)) cwk /= 2;
-        if (q)
-            while (tk > 2 && krow_bad(tk, kLdsTile)) tk /= 2;
-        if (row_bad(tz, kLdsMax) || col_bad(cwk, kLdsMax) || (q && krow_bad(tk, kLdsMax)))
-            return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", i, j.nxx, j.nzz);
-        maxnxx = std::max(maxnxx, j.nxx);
-        maxnzz = std::max(maxnzz, j.nzz);
-    }
-    b->tz = tz;
-    b->cw = cw;
-    b->cwq = cwq;
-    b->tk = tk;
-    b->snaps = b->rtm ? jobs[0].snaps : 0;
+    std::vector<TileReq> req;
+    for (int i = 0; i < njobs; i++) req.push_back({jobs[i].nxx, jobs[i].nzz, is_q(jobs[i].kind)});
+    int rc = choose_tiles(b.get(), req);
+    if (rc) return rc;
     b->jobs.resize(njobs);
-    std::vector<int2> items[2][P_COUNT];
     for (int i = 0; i < njobs; i++) {
-        int rc = build_job(b.get(), jobs[i], b->jobs[i]);
+        rc = build_job(b.get(), jobs[i], b->jobs[i]);
         if (rc) return rc;
-        const JobH &h = b->jobs[i];
-        b->tmax = std::max(b->tmax, h.nt);
-        for (int r = 0; r < h.nroles; r++) {
-            std::vector<int2> *it = items[r];
-            if (!is_q(h.kind)) {
-                for (int z0 = 0; z0 < h.nzz; z0 += tz) it[P_AROW].push_back(make_int2(i, z0));
-                for (int x0 = 0; x0 < h.nxx; x0 += cw) it[P_ACOL].push_back(make_int2(i, x0));
-            } else {
-                for (int x0 = 0; x0 < h.nxx; x0 += cwq) {
-                    it[P_QC1].push_back(make_int2(i, x0));
-                    it[P_QC2].push_back(make_int2(i, x0));
-                    it[P_QC3].push_back(make_int2(i, x0));
-                }
-                for (int k0 = 0; k0 < h.d[0].nzh; k0 += tk) {
-                    it[P_QR1].push_back(make_int2(i, k0));
-                    it[P_QR2].push_back(make_int2(i, k0));
-                }
-                for (int z0 = 0; z0 < h.nzz; z0 += tz) it[P_QR3].push_back(make_int2(i, z0));
-            }
-        }
     }
-    const size_t lrow = lds_lines(maxnxx, tz / 2);
-    const size_t lcol = lds_lines(maxnzz, cw / 2);
-    b->lds[P_AROW] = lrow;
-    b->lds[P_QR3] = lrow;
-    b->lds[P_ACOL] = lcol;
-    b->lds[P_QC1] = lds_lines(maxnzz, cwq / 2);
-    b->lds[P_QC2] = lds_col_acc(maxnzz, cwq);
-    b->lds[P_QC3] = lds_col_acc(maxnzz, cwq);
-    b->lds[P_QR1] = lds_lines(maxnxx, 3 * tk);
-    b->lds[P_QR2] = lds_lines(maxnxx, tk);
-    for (int p = 0; p < P_COUNT; p++)
-        if (!items[0][p].empty() && b->lds[p] > kLdsMax)
-            return set_err(CQ_EUNSUPPORTED, "pass %s needs %zu B of LDS", kPassName[p], b->lds[p]);
-    // job table: role 0 entries, then role 1 entries (same index + njobs)
-    std::vector<JobD> hd(2 * (size_t)njobs);
-    for (int i = 0; i < njobs; i++) {
-        hd[i] = b->jobs[i].d[0];
-        hd[njobs + i] = b->jobs[i].nroles > 1 ? b->jobs[i].d[1] : b->jobs[i].d[0];
-    }
-    HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * hd.size()));
-    HIPCHK(hipMemcpy(b->djobs, hd.data(), sizeof(JobD) * hd.size(), hipMemcpyHostToDevice));
-    for (int r = 0; r < 2; r++)
-        for (int p = 0; p < P_COUNT; p++) {
-            if (items[r][p].empty()) continue;
-            std::vector<int2> o = xcd_order(items[r][p]);
-            HIPCHK(hipMalloc(&b->ditems[r][p], sizeof(int2) * o.size()));
-            HIPCHK(hipMemcpy(b->ditems[r][p], o.data(), sizeof(int2) * o.size(), hipMemcpyHostToDevice));
-            b->nitems[r][p] = (int)o.size();
-        }
+    rc = rebuild_tables(b.get());
+    if (rc) return rc;
     *out = b.release();
+    ctx->batches.insert(*out);
     return CQ_OK;
 }
 
 template <typename K>
-static int launch(cq_batch b, K kernel, int pass, int role, int step, int t_aux, int flags)
+static int launch(cq_batch b, K kernel, int pass, int step)
 {
     const size_t lds = b->lds[pass];
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const JobD *table = b->djobs + (role ? b->jobs.size() : 0);
-    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, b->ctx->stream, table,
-                       b->ditems[role][pass], step, t_aux, flags);
+    hipLaunchKernelGGL(kernel, dim3(b->nitems[pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs,
+                       b->ditems[pass], step);
     HIPCHK(hipGetLastError());
     return CQ_OK;
 }
 
-#define CQ_ARGS pass, role, step, t_aux, flags
+#define CQ_ARGS pass, step
 #define CQ_DISPATCH_TZ(K)                                                          \
-    (b->tz == 16 ? launch(b, K<16>, CQ_ARGS) : launch(b, K<8>, CQ_ARGS))
+    (b->tz == 16   ? launch(b, K<16>, CQ_ARGS)                                     \
+     : b->tz == 12 ? launch(b, K<12>, CQ_ARGS) : launch(b, K<8>, CQ_ARGS))
 #define CQ_DISPATCH_CW(K, W)                                                       \
     (W == 32   ? launch(b, K<32>, CQ_ARGS)                                         \
+     : W == 24 ? launch(b, K<24>, CQ_ARGS)                                         \
      : W == 16 ? launch(b, K<16>, CQ_ARGS)                                         \
+     : W == 12 ? launch(b, K<12>, CQ_ARGS)                                         \
      : W == 8  ? launch(b, K<8>, CQ_ARGS)                                          \
      : W == 4  ? launch(b, K<4>, CQ_ARGS)                                          \
                : launch(b, K<2>, CQ_ARGS))
@@ -753,7 +880,7 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step, int t_aux,
     (b->tk == 8 ? launch(b, K<8>, CQ_ARGS)                                         \
      : b->tk == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
 
-static int launch_pass(cq_batch b, int pass, int role, int step, int t_aux, int flags)
+static int launch_pass(cq_batch b, int pass, int step)
 {
     switch (pass) {
     case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
@@ -768,8 +895,8 @@ static int launch_pass(cq_batch b, int pass, int role, int step, int t_aux, int 
     return set_err(CQ_EINVAL, "unknown pass %d", pass);
 }
 
-// One time step of every wavefield of the given role.
-static int one_step(cq_batch b, int role, int step, int t_aux, int flags)
+// Global step `step` of every wavefield of the batch (one launch per pass).
+static int one_step(cq_batch b, int step)
 {
     if (b->timing) {
         hipEvent_t e0;
@@ -779,8 +906,8 @@ static int one_step(cq_batch b, int role, int step, int t_aux, int flags)
         b->ev_pass.push_back(-1);
     }
     for (int p = 0; p < P_COUNT; p++) {
-        if (b->nitems[role][p] == 0) continue;
-        int rc = launch_pass(b, p, role, step, t_aux, flags);
+        if (b->nitems[p] == 0) continue;
+        int rc = launch_pass(b, p, step);
         if (rc) return rc;
         if (b->timing) {
             hipEvent_t e1;
@@ -793,17 +920,56 @@ static int one_step(cq_batch b, int role, int step, int t_aux, int flags)
     return CQ_OK;
 }
 
+__global__ void k_add_point(float *p, float v) { *p += v; }
+
+// One checkpoint copy of an RTM shot: dir 1 state -> slot, 0 slot -> state
+// (storecheckpoints, sgpstd-qrtm/acps4.cu:1797-1844).  A slot taken at capo 0
+// also gets the reference's wavelet sample rik[0] dt 1000 added at the
+// source (addsource, :1846-1863).
+static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
+{
+    hipStream_t s = b->ctx->stream;
+    char *ck = h.ckpt + (size_t)op.slot * h.state_bytes;
+    if (op.dir) {
+        HIPCHK(hipMemcpyAsync(ck, h.state[0], h.state_bytes, hipMemcpyDeviceToDevice, s));
+        if (op.src0) {
+            const size_t off = (size_t)((char *)h.d[0].p - h.state[0]);
+            hipLaunchKernelGGL(k_add_point, dim3(1), dim3(1), 0, s,
+                               reinterpret_cast<float *>(ck + off) + h.src_cell, h.src0);
+            HIPCHK(hipGetLastError());
+        }
+    } else {
+        HIPCHK(hipMemcpyAsync(h.state[0], ck, h.state_bytes, hipMemcpyDeviceToDevice, s));
+    }
+    return CQ_OK;
+}
+
+// Run global steps b->t .. end - 1; before each, the checkpoint copies that
+// the RTM shots' schedules place there.
+static int advance_to(cq_batch b, int end)
+{
+    for (; b->t < end; b->t++) {
+        for (auto &h : b->jobs) {
+            const int s = b->t - h.t0;
+            while (h.op_next < h.ops.size() && h.ops[h.op_next].at <= s) {
+                if (h.ops[h.op_next].at == s) {
+                    int rc = ckpt_apply(b, h, h.ops[h.op_next]);
+                    if (rc) return rc;
+                }
+                h.op_next++;
+            }
+        }
+        int rc = one_step(b, b->t);
+        if (rc) return rc;
+    }
+    return CQ_OK;
+}
+
 extern "C" int cq_batch_step(cq_batch b, int nsteps)
 {
     if (!b || nsteps < 0) return set_err(CQ_EINVAL, "cq_batch_step: bad arguments");
-    if (b->rtm) return set_err(CQ_ESTATE, "cq_batch_step: RTM batches run with cq_batch_run");
     HIPCHK(hipSetDevice(b->ctx->device));
-    for (int s = 0; s < nsteps && b->t < b->tmax; s++) {
-        int rc = one_step(b, 0, b->t, b->t - 1, 0);
-        if (rc) return rc;
-        b->t++;
-    }
-    return CQ_OK;
+    return advance_to(b, std::min(b->tmax, b->t + nsteps));
 }
 
 extern "C" int cq_batch_sync(cq_batch b)
@@ -813,98 +979,43 @@ extern "C" int cq_batch_sync(cq_batch b)
     return CQ_OK;
 }
 
-__global__ void k_add_point(float *p, float v) { *p += v; }
-
-// Checkpoint copies for every shot: dir 1 state -> slot, 0 slot -> state
-// (storecheckpoints, sgpstd-qrtm/acps4.cu:1797-1844).  Slot 0 also gets the
-// reference's wavelet sample rik[0] dt 1000 added at the source
-// (addsource, :1846-1863) when taken at capo 0.
-static int checkpoint(cq_batch b, int dir, int slot, bool add_src0)
-{
-    hipStream_t s = b->ctx->stream;
-    for (auto &h : b->jobs) {
-        char *ck = h.ckpt + (size_t)slot * h.state_bytes;
-        if (dir) {
-            HIPCHK(hipMemcpyAsync(ck, h.state[0], h.state_bytes, hipMemcpyDeviceToDevice, s));
-            if (add_src0) {
-                const size_t off = (size_t)((char *)h.d[0].p - h.state[0]);
-                hipLaunchKernelGGL(k_add_point, dim3(1), dim3(1), 0, s,
-                                   reinterpret_cast<float *>(ck + off) + h.src_cell, h.src0);
-                HIPCHK(hipGetLastError());
-            }
-        } else {
-            HIPCHK(hipMemcpyAsync(h.state[0], ck, h.state_bytes, hipMemcpyDeviceToDevice, s));
-        }
-    }
-    return CQ_OK;
-}
-
-// The revolve-driven RTM loop (sgpstd-qrtm/acps4.cu:447-905,
-// sgpstd-qrtm/acsglw.cu:470-1040) for every shot of the batch in lock step.
-static int rtm_run(cq_batch b)
-{
-    if (b->rtm_done) return CQ_OK;
-    cqh_revolve rv;
-    cqh_revolve_init(&rv, b->tmax, b->snaps);
-    int ts = 0, restart = 0, rc = CQ_OK;
-    for (;;) {
-        const int a = cqh_revolve_next(&rv);
-        if (a == CQH_RV_TAKESHOT) {
-            rc = checkpoint(b, 1, rv.check, rv.capo == 0);
-        } else if (a == CQH_RV_ADVANCE) {
-            for (int t = ts; t <= rv.capo && rc == CQ_OK; t++)
-                rc = one_step(b, 0, t, t - 1, restart ? 0 : CQ_F_SSG);
-            ts = rv.capo + 1;
-        } else if (a == CQH_RV_FIRSTURN) {
-            restart = 1;
-            rc = one_step(b, 1, b->tmax - 1, b->tmax, 0);
-        } else if (a == CQH_RV_YOUTURN) {
-            rc = one_step(b, 1, rv.capo, rv.capo + 1, 0);
-        } else if (a == CQH_RV_RESTORE) {
-            rc = checkpoint(b, 0, rv.check, false);
-            ts = rv.capo + 1;
-        } else if (a == CQH_RV_TERMINATE) {
-            break;
-        } else {
-            b->rv_info = rv.info;
-            return set_err(CQ_EINVAL, "revolve error, info %d", rv.info);
-        }
-        if (rc) return rc;
-    }
-    b->rtm_done = 1;
-    b->t = b->tmax;
-    return CQ_OK;
-}
-
 extern "C" int cq_batch_run(cq_batch b)
 {
     if (!b) return set_err(CQ_EINVAL, "cq_batch_run: null batch");
     HIPCHK(hipSetDevice(b->ctx->device));
-    int rc = b->rtm ? rtm_run(b) : cq_batch_step(b, b->tmax - b->t);
+    int rc = advance_to(b, b->tmax);
     if (rc) return rc;
     return cq_batch_sync(b);
+}
+
+static int read_image(cq_ctx ctx, const JobH &h, float *img, float *ssg)
+{
+    if (!is_rtm(h.kind)) return set_err(CQ_ESTATE, "not an RTM shot");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(img, h.d[0].img, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ssg, h.d[0].ssg, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
+    return CQ_OK;
+}
+
+static int read_rec(cq_ctx ctx, const JobH &h, float *rec)
+{
+    if (is_rtm(h.kind)) return set_err(CQ_ESTATE, "RTM shots have no gathers");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (h.nr > 0) HIPCHK(hipMemcpy(rec, h.d[0].rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
+    return CQ_OK;
 }
 
 extern "C" int cq_batch_read_image(cq_batch b, int job, float *img, float *ssg)
 {
     if (!b || job < 0 || job >= (int)b->jobs.size() || !img || !ssg)
         return set_err(CQ_EINVAL, "cq_batch_read_image: bad arguments");
-    if (!b->rtm) return set_err(CQ_ESTATE, "cq_batch_read_image: not an RTM batch");
-    const JobH &h = b->jobs[job];
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    HIPCHK(hipMemcpy(img, h.d[0].img, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ssg, h.d[0].ssg, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
-    return CQ_OK;
+    return read_image(b->ctx, b->jobs[job], img, ssg);
 }
 
 extern "C" int cq_batch_read_rec(cq_batch b, int job, float *rec)
 {
     if (!b || job < 0 || job >= (int)b->jobs.size() || !rec) return set_err(CQ_EINVAL, "cq_batch_read_rec: bad arguments");
-    const JobH &h = b->jobs[job];
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    if (b->rtm) return set_err(CQ_ESTATE, "cq_batch_read_rec: RTM batches have no gathers");
-    if (h.nr > 0) HIPCHK(hipMemcpy(rec, h.d[0].rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
-    return CQ_OK;
+    return read_rec(b->ctx, b->jobs[job], rec);
 }
 
 extern "C" int cq_batch_read_p(cq_batch b, int job, float *p)
@@ -927,13 +1038,281 @@ extern "C" long long cq_batch_bytes(cq_batch b)
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// serving: continuous batching over one device-resident batch
+// ---------------------------------------------------------------------------
+// A submitted job owns copies of all its host arrays until it is admitted.
+struct ServerJob {
+    int id = 0, state = CQ_JOB_QUEUED;
+    cq_job_desc d;
+    std::vector<float> f[32];
+    std::vector<int> gx, gz;
+    int t_submit = 0, t_start = -1, t_end = -1;
+    size_t bytes = 0;
+    std::vector<float> out, ssg;   // results (gather, or image + illumination)
+};
+
+struct cq_server_s {
+    cq_batch_s b;
+    cq_server_desc desc;
+    size_t budget = 0, resident = 0;
+    int next_id = 1;
+    std::map<int, std::unique_ptr<ServerJob>> jobs;
+    std::vector<int> queue;         // ids waiting for admission, FIFO
+    std::vector<int> running;       // ids of b.jobs, same order
+};
+
+// Tile shapes fixed for every window up to max_nxx x max_nzz: the largest
+// tiles whose LDS fits the budget and whose point count fits the smallest
+// per-stage capacity of the power-of-two radices.
+static int server_tiles(cq_batch b, int mx, int mz)
+{
+    const int cap = cq_stage_capacity(16);
+    int iz = 0, ic = 0, icq = 0, ik = 0;
+    auto lim = [&](size_t lds, int pts) { return lds > kLdsTile || pts > cap; };
+    while (iz < 2 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx)) iz++;
+    while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz)) ic++;
+    icq = ic;
+    while (ik < 2 && lim(lds_lines(mx, 3 * kTkCand[ik]), 3 * kTkCand[ik] * mx)) ik++;
+    b->tz = kTzCand[iz];
+    b->cw = kCwCand[ic];
+    b->cwq = kCwCand[icq];
+    b->tk = kTkCand[ik];
+    if (lds_lines(mx, b->tz / 2) > kLdsMax || lds_lines(mz, b->cw / 2) > kLdsMax ||
+        lds_lines(mx, 3 * b->tk) > kLdsMax || (b->tz / 2) * mx > cap || 3 * b->tk * mx > cap)
+        return set_err(CQ_EUNSUPPORTED, "server windows up to %dx%d are too large for one workgroup", mx, mz);
+    b->maxnxx = mx;
+    b->maxnzz = mz;
+    set_lds(b, mx, mz, mx, mz);
+    return CQ_OK;
+}
+
+extern "C" int cq_server_open(cq_ctx ctx, const cq_server_desc *desc, cq_server *out)
+{
+    if (!ctx || !desc || !out || desc->max_nxx < 8 || desc->max_nzz < 8 || desc->hbm_budget < 0 || desc->max_jobs < 0)
+        return set_err(CQ_EINVAL, "cq_server_open: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<cq_server_s> s(new cq_server_s());
+    s->b.ctx = ctx;
+    s->desc = *desc;
+    int rc = server_tiles(&s->b, desc->max_nxx, desc->max_nzz);
+    if (rc) return rc;
+    if (desc->hbm_budget > 0) {
+        s->budget = (size_t)desc->hbm_budget;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        s->budget = fr / 10 * 9;
+    }
+    *out = s.release();
+    ctx->servers.insert(*out);
+    return CQ_OK;
+}
+
+// Deep copy of a job description (arrays the engine reads).
+static void copy_job(ServerJob &sj, const cq_job_desc &j)
+{
+    sj.d = j;
+    const size_t nc = (size_t)j.nxx * j.nzz;
+    int k = 0;
+    auto own = [&](const float *&p, size_t n) {
+        if (!p) return;
+        sj.f[k].assign(p, p + n);
+        p = sj.f[k].data();
+        k++;
+    };
+    own(sj.d.vp, nc); own(sj.d.rho, nc); own(sj.d.bbx, nc); own(sj.d.bbz, nc);
+    own(sj.d.gam, nc); own(sj.d.lam, nc); own(sj.d.vwd, nc); own(sj.d.tao, nc);
+    own(sj.d.iax, j.nxx); own(sj.d.ibx, j.nxx); own(sj.d.hax, j.nxx); own(sj.d.hbx, j.nxx);
+    own(sj.d.iaz, j.nzz); own(sj.d.ibz, j.nzz); own(sj.d.haz, j.nzz); own(sj.d.hbz, j.nzz);
+    own(sj.d.kxsmr, j.nxx); own(sj.d.kxsmi, j.nxx); own(sj.d.kxscr, j.nxx); own(sj.d.kxsci, j.nxx);
+    own(sj.d.kzsmr, j.nzz); own(sj.d.kzsmi, j.nzz); own(sj.d.kzscr, j.nzz); own(sj.d.kzsci, j.nzz);
+    own(sj.d.rik, (size_t)j.nt);
+    if (is_rtm(j.kind) && j.nr > 0) own(sj.d.obs, (size_t)j.nt * j.nr);
+    else sj.d.obs = nullptr;
+    if (j.nr > 0) {
+        sj.gx.assign(j.gxg, j.gxg + j.nr);
+        sj.gz.assign(j.gzg, j.gzg + j.nr);
+        sj.d.gxg = sj.gx.data();
+        sj.d.gzg = sj.gz.data();
+    }
+}
+
+extern "C" int cq_server_submit(cq_server s, const cq_job_desc *job, int *id)
+{
+    if (!s || !job || !id) return set_err(CQ_EINVAL, "cq_server_submit: bad arguments");
+    int rc = validate(*job, 0);
+    if (rc) return rc;
+    if (job->nxx > s->desc.max_nxx || job->nzz > s->desc.max_nzz)
+        return set_err(CQ_EUNSUPPORTED, "window %dx%d exceeds the server's %dx%d", job->nxx, job->nzz,
+                       s->desc.max_nxx, s->desc.max_nzz);
+    HIPCHK(hipSetDevice(s->b.ctx->device));
+    std::unique_ptr<ServerJob> sj(new ServerJob());
+    copy_job(*sj, *job);
+    JobH probe;
+    rc = build_job(&s->b, sj->d, probe, &sj->bytes);   // plans exist, footprint
+    if (rc) return rc;
+    if (sj->bytes > s->budget)
+        return set_err(CQ_ENOMEM, "job needs %zu device bytes, the server's budget is %zu", sj->bytes, s->budget);
+    sj->id = s->next_id++;
+    sj->t_submit = s->b.t;
+    *id = sj->id;
+    s->queue.push_back(sj->id);
+    s->jobs[sj->id] = std::move(sj);
+    return CQ_OK;
+}
+
+// Copies finished jobs' results to the host and frees their device memory;
+// then admits queued jobs (FIFO) that fit the budget and the job limit.  The
+// stream is idle on return when anything changed, and the tables are rebuilt.
+static int server_turnover(cq_server s)
+{
+    cq_batch b = &s->b;
+    bool changed = false;
+    for (size_t i = 0; i < b->jobs.size();) {
+        JobH &h = b->jobs[i];
+        if (b->t < h.t0 + h.nsched) {
+            i++;
+            continue;
+        }
+        ServerJob &sj = *s->jobs[s->running[i]];
+        if (is_rtm(h.kind)) {
+            sj.out.resize((size_t)h.nx * h.nz);
+            sj.ssg.resize((size_t)h.nx * h.nz);
+            int rc = read_image(b->ctx, h, sj.out.data(), sj.ssg.data());
+            if (rc) return rc;
+        } else {
+            sj.out.resize((size_t)h.nt * h.nr);
+            int rc = read_rec(b->ctx, h, sj.out.data());
+            if (rc) return rc;
+        }
+        sj.state = CQ_JOB_DONE;
+        sj.t_end = b->t;
+        HIPCHK(hipFree(h.mem));
+        s->resident -= sj.bytes;
+        b->jobs.erase(b->jobs.begin() + i);
+        s->running.erase(s->running.begin() + i);
+        changed = true;
+    }
+    while (!s->queue.empty()) {
+        ServerJob &sj = *s->jobs[s->queue.front()];
+        if (s->resident + sj.bytes > s->budget) break;
+        if (s->desc.max_jobs > 0 && (int)b->jobs.size() >= s->desc.max_jobs) break;
+        if (!changed) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        b->jobs.emplace_back();
+        JobH &h = b->jobs.back();
+        int rc = build_job(b, sj.d, h);
+        if (rc) {
+            b->jobs.pop_back();
+            return rc;
+        }
+        h.t0 = b->t;
+        for (int r = 0; r < h.nroles; r++) h.d[r].t0 = b->t;
+        s->resident += sj.bytes;
+        s->running.push_back(sj.id);
+        s->queue.erase(s->queue.begin());
+        sj.state = CQ_JOB_RUNNING;
+        sj.t_start = b->t;
+        for (auto &v : sj.f) std::vector<float>().swap(v);   // inputs now live on the device
+        changed = true;
+    }
+    if (changed) return rebuild_tables(b);
+    return CQ_OK;
+}
+
+extern "C" int cq_server_advance(cq_server s, int nsteps, int *running, int *queued)
+{
+    if (!s || nsteps < 0) return set_err(CQ_EINVAL, "cq_server_advance: bad arguments");
+    cq_batch b = &s->b;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const int end = b->t + nsteps;
+    int rc = server_turnover(s);
+    while (rc == CQ_OK && b->t < end && !b->jobs.empty()) {
+        // run to the next finishing job (or the end), then turn over
+        int next = end;
+        for (auto &h : b->jobs) next = std::min(next, h.t0 + h.nsched);
+        rc = advance_to(b, std::max(next, b->t + 1));
+        if (rc == CQ_OK && b->t >= next) {
+            HIPCHK(hipStreamSynchronize(b->ctx->stream));
+            rc = server_turnover(s);
+        }
+    }
+    if (rc) return rc;
+    if (running) *running = (int)b->jobs.size();
+    if (queued) *queued = (int)s->queue.size();
+    return CQ_OK;
+}
+
+static ServerJob *server_job(cq_server s, int id)
+{
+    auto it = s->jobs.find(id);
+    return it == s->jobs.end() ? nullptr : it->second.get();
+}
+
+extern "C" int cq_server_poll(cq_server s, int id, int *state)
+{
+    ServerJob *sj = s ? server_job(s, id) : nullptr;
+    if (!sj || !state) return set_err(CQ_EINVAL, "cq_server_poll: unknown job %d", id);
+    *state = sj->state;
+    return CQ_OK;
+}
+
+extern "C" int cq_server_job_steps(cq_server s, int id, int *submitted, int *started, int *finished)
+{
+    ServerJob *sj = s ? server_job(s, id) : nullptr;
+    if (!sj) return set_err(CQ_EINVAL, "cq_server_job_steps: unknown job %d", id);
+    if (submitted) *submitted = sj->t_submit;
+    if (started) *started = sj->t_start;
+    if (finished) *finished = sj->t_end;
+    return CQ_OK;
+}
+
+extern "C" int cq_server_result(cq_server s, int id, float *out, float *ssg)
+{
+    ServerJob *sj = s ? server_job(s, id) : nullptr;
+    if (!sj || !out) return set_err(CQ_EINVAL, "cq_server_result: bad arguments");
+    if (sj->state != CQ_JOB_DONE) return set_err(CQ_ESTATE, "job %d has not finished", id);
+    memcpy(out, sj->out.data(), sizeof(float) * sj->out.size());
+    if (!sj->ssg.empty()) {
+        if (!ssg) return set_err(CQ_EINVAL, "cq_server_result: RTM job %d needs ssg", id);
+        memcpy(ssg, sj->ssg.data(), sizeof(float) * sj->ssg.size());
+    }
+    return CQ_OK;
+}
+
+extern "C" int cq_server_release(cq_server s, int id)
+{
+    ServerJob *sj = s ? server_job(s, id) : nullptr;
+    if (!sj) return set_err(CQ_EINVAL, "cq_server_release: unknown job %d", id);
+    if (sj->state != CQ_JOB_DONE) return set_err(CQ_ESTATE, "job %d has not finished", id);
+    s->jobs.erase(id);
+    return CQ_OK;
+}
+
+extern "C" int cq_server_close(cq_server s)
+{
+    if (!s) return CQ_OK;
+    (void)hipSetDevice(s->b.ctx->device);
+    (void)hipStreamSynchronize(s->b.ctx->stream);
+    s->b.ctx->servers.erase(s);
+    delete s;
+    return CQ_OK;
+}
+
 extern "C" int cq_batch_destroy(cq_batch b)
 {
     if (!b) return CQ_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    b->ctx->batches.erase(b);
     delete b;
     return CQ_OK;
+}
+
+static void destroy_children(cq_ctx ctx)
+{
+    while (!ctx->servers.empty()) cq_server_close(*ctx->servers.begin());
+    while (!ctx->batches.empty()) cq_batch_destroy(*ctx->batches.begin());
 }
 
 extern "C" int cq_batch_enable_timing(cq_batch b, int on)

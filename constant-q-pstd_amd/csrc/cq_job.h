@@ -24,6 +24,10 @@
     float eta, taoc, dt; /* stabiliser (sgpstd-qrtm/acsglw.cu:1895-1924) */                       \
     float src_scale;   /* dt * 1000 (acps4.cu:1491) */                                            \
     float dkx, dkz, invN; /* wavenumber steps (funca.c:555-556) and 1/(nxx nzz) */                \
+    /* steps: the wavefield takes part in global steps t0 .. t0 + nsched - 1; sched[s] = */       \
+    /* {role, t, t_aux, flags} of its step s (RTM), or none: t = s, t_aux = s - 1 (modeling) */   \
+    int t0, nsched;                                                                               \
+    const int4 G *sched;                                                                          \
     CqPlan px, pz;                                                                                \
     const float2 G *twx;                                                                          \
     const float2 G *twz;                                                                          \
@@ -33,6 +37,7 @@
     /* wavefield state */                                                                         \
     float G *p; float G *vx; float G *vz; float G *div;                                           \
     float G *lz; float G *dvx; float G *dvz; float G *tot;                                        \
+    float G *scr;      /* per-cell scratch of the viscoacoustic column pass C3 */                 \
     /* per-cell coefficients; acoustic: rho v^2 dt, rho v^4 dt^3/12 */                            \
     const float G *c1; const float G *c2; const float G *dtbx; const float G *dtbz;               \
     /* viscoacoustic (see build_job) */                                                           \

@@ -123,15 +123,6 @@ __device__ __forceinline__ float *col_line(float *bufs, int nzz, int c)
     return bufs + 2 * (c >> 1) * cq_line_stride(nzz) + (c & 1);
 }
 
-// Per-column accumulator of a column tile: one float4 per (column c, depths
-// 4 zq .. 4 zq + 3) at index c * nq + zq, placed 16-byte aligned after the
-// twiddle table and the L packed lines.
-__device__ __forceinline__ float4 *col_acc(float2 *lds, int nzz, int L)
-{
-    const int off = nzz + L * cq_line_stride(nzz);
-    return reinterpret_cast<float4 *>(lds + ((off + 1) & ~1));
-}
-
 template <int CW>
 __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float CQ_GLOBAL *__restrict__ src, int x0)
 {
@@ -379,16 +370,38 @@ __device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, in
 // ===========================================================================
 // acoustic passes (sgpstd-qmodeling/acps4.cu:508-645)
 // ===========================================================================
-// Kernel arguments: step = time index of this step (wavelet sample for the
-// source wavefield, data sample for the receiver wavefield); t_aux = time
-// index the stabilised aux multiplier uses; flags = CQ_F_* bits.
-#define CQ_ITEM_PROLOGUE                      \
-    const int2 it = items[blockIdx.x];        \
-    if (it.x < 0) return;                     \
-    const JobG &J = reinterpret_cast<const JobG *>(jobs)[it.x]; \
-    const int t = step;                       \
-    if (t >= J.nt) return;                    \
-    (void)t_aux;                              \
+// Kernel arguments: the job table, the work items (table entry, tile origin)
+// and the global step.  Each wavefield finds its own step record: t = time
+// index of this step (wavelet sample for the source wavefield, data sample
+// for the receiver wavefield), t_aux = time index the stabilised aux
+// multiplier uses, flags = CQ_F_* bits.  It returns false when the wavefield
+// does not advance at this global step (not started, finished, or the other
+// role's turn in an RTM schedule).
+__device__ __forceinline__ bool cq_step_of(const JobG &J, int step, int &t, int &t_aux, int &flags)
+{
+    const int s = step - J.t0;
+    if (s < 0 || s >= J.nsched) return false;
+    if (J.sched == nullptr) {
+        t = s;
+        t_aux = s - 1;
+        flags = 0;
+        return true;
+    }
+    const int4 r = J.sched[s];
+    if (r.x != J.role) return false;
+    t = r.y;
+    t_aux = r.z;
+    flags = r.w;
+    return true;
+}
+
+#define CQ_ITEM_PROLOGUE                                             \
+    const int2 it = items[blockIdx.x];                               \
+    if (it.x < 0) return;                                            \
+    const JobG &J = reinterpret_cast<const JobG *>(jobs)[it.x];      \
+    int t, t_aux, flags;                                             \
+    if (!cq_step_of(J, step, t, t_aux, flags)) return;               \
+    (void)t_aux;                                                     \
     (void)flags;
 
 // dst[interior(ix, iz0 + k)] += a.k * b.k for the 4 depths of a float4
@@ -408,7 +421,7 @@ __device__ __forceinline__ void interior_accum(const JobG &J, float CQ_GLOBAL *d
 // row pass: L = Lx + Lz, p update + source, dp/dx -> vx, dvx/dx -> dvx
 template <int TZ>
 __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                  int step, int t_aux, int flags)
+                                                  int step)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
@@ -481,7 +494,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
 // column pass: gather, dp/dz -> vz, dvz/dz -> div, Lz = d2/dz2 div
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                  int step, int t_aux, int flags)
+                                                  int step)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -531,7 +544,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
 // ===========================================================================
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -548,7 +561,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
 
 template <int TK>
 __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y, LS = cq_line_stride(nxx);
@@ -579,7 +592,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
 
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -587,10 +600,11 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
-    float4 *acc = col_acc(lds, nzz, L);
     load_tw(tw, J.twz, nzz);
     // total = y1 div + y2 A - (dt^2 v^2 lam/12)(y3 K + y4 B), with
-    // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2
+    // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2.
+    // The partial sums live in `tot` itself; each thread re-reads only the
+    // elements it wrote (they stay in L2).
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
@@ -598,14 +612,15 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
-        const float4 g = ld4(J.qg + (size_t)ix * ldz + 4 * zq);
+        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const float4 gm = ld4(J.qg + g);
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(g, k)) * line[2 * cq_pad(iz)]);
+            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(gm, k)) * line[2 * cq_pad(iz)]);
         }
-        acc[e] = a;
+        st4(J.tot + g, a);
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s4);   // B
@@ -617,13 +632,13 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
         const float *line = col_line(bufs, nzz, c);
         const size_t g = (size_t)ix * ldz + 4 * zq;
         const float4 y2 = ld4(J.qy2 + g), cc = ld4(J.qcc + g);
-        float4 a = acc[e];
+        float4 a = ld4(J.tot + g);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * cq_pad(iz)]));
         }
-        acc[e] = a;
+        st4(J.tot + g, a);
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s1);   // A
@@ -639,7 +654,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
             continue;
         }
         const size_t g = (size_t)ix * ldz + 4 * zq;
-        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g), a = acc[e];
+        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g), a = ld4(J.tot + g);
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -659,7 +674,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 
 template <int TK>
 __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y;
@@ -677,7 +692,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, c
 
 template <int CW>
 __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
@@ -685,15 +700,14 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
-    float4 *acc = col_acc(lds, nzz, L);
-    float *accf = reinterpret_cast<float *>(acc);   // accf[c * 4 nq + iz]
-    const int AS = 4 * nq;
     load_tw(tw, J.twz, nzz);
     col_insert<CW>(J, buf, x0, J.s1);   // T
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    // T waits in the scratch field (each thread re-reads only what it wrote)
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq;
+        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -701,7 +715,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
             const int iz = 4 * zq + k;
             if (iz < nzz) f4set(a, k, line[2 * cq_pad(iz)]);
         }
-        acc[e] = a;
+        st4(J.scr + (size_t)ix * ldz + 4 * zq, a);
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
@@ -709,18 +723,23 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     // p += rho v^2 dt lam (total - s dt tao T / 2) - s dt tao aux; source.
     // s = +1 attenuates (acsglw.cu:1500-1537); s = -1 compensates (Q-RTM,
-    // sgpstd-qrtm/acsglw.cu:1769-1841)
+    // sgpstd-qrtm/acsglw.cu:1769-1841).  The new p replaces aux in the
+    // packed columns (each thread overwrites the values it read).
     const bool rcv = J.role == 1;
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     const float s = J.qsign;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
-        if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
+        float *line = col_line(bufs, nzz, c);
+        if (ix >= J.nxx) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
+            continue;
+        }
         const size_t g = (size_t)ix * ldz + 4 * zq;
         const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
-                     qe = ld4(J.qe + g), tt = acc[e];
-        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+                     qe = ld4(J.qe + g), tt = ld4(J.scr + g);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
@@ -728,9 +747,8 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
             float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * f4get(tt, k)) -
                       s * f4get(qe, k) * line[2 * cq_pad(iz)];
             if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
-            f4set(pn, k, v);
+            line[2 * cq_pad(iz)] = v;
         }
-        acc[e] = pn;
     }
     __syncthreads();
     if (rcv) {
@@ -738,24 +756,23 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         // sgpstd-qrtm/acsglw.cu:1809-1841)
         const int tile = x0 / CW;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT)
-            accf[(J.inj_x[r] - x0) * AS + J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+            col_line(bufs, nzz, J.inj_x[r] - x0)[2 * cq_pad(J.inj_z[r])] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         __syncthreads();
     }
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int c = e / nq, zq = e - c * nq, ix = x0 + c;
         if (ix >= J.nxx) continue;
+        const float *line = col_line(bufs, nzz, c);
         const size_t g = (size_t)ix * ldz + 4 * zq;
-        const float4 pn = acc[e];
+        float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (4 * zq + k < nzz) f4set(pn, k, line[2 * cq_pad(4 * zq + k)]);
         st4(J.p + g, pn);
         if (rcv) interior_accum(J, J.img, ix, 4 * zq, pn, ld4(J.psrc + g));
         else if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, 4 * zq, pn, pn);
     }
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return accf[c * AS + z]; });
-    // p -> LDS (packed columns) for F_z p and dp/dz
-    for (int e = threadIdx.x; e < CW * nzz; e += CQ_NT) {
-        const int c = e / nzz, iz = e - c * nzz;
-        col_line(bufs, nzz, c)[2 * cq_pad(iz)] = (x0 + c < J.nxx) ? accf[c * AS + iz] : 0.f;
-    }
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * cq_pad(z)]; });
     __syncthreads();
     cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
@@ -770,7 +787,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
 
 template <int TZ>
 __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
-                                                int step, int t_aux, int flags)
+                                                int step)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE

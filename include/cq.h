@@ -81,30 +81,68 @@ const char *cq_last_error(void);
 const char *cq_version(void);
 
 /* Uploads njobs shots (models, coefficients, receiver maps) and zeroes their
- * state.  Shots of different sizes and kinds may be mixed. */
+ * state.  Shots of different sizes and kinds (modeling, RTM, Q-RTM) may be
+ * mixed; every launch advances all of them.  Replaces the per-shot
+ * acforward calls of sgpstd-qmodeling/acps4.cu:184-198, acsglw.cu:185-200,
+ * sgpstd-qrtm/acps4.cu:183-198 and acsglw.cu:189-204. */
 int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out);
-/* Enqueues nsteps time steps of every unfinished shot (asynchronous; shots
- * whose nt is reached are skipped). */
+/* Enqueues nsteps global steps (asynchronous).  A modeling shot takes one
+ * time step per global step until its nt; an RTM shot takes the next step of
+ * its own revolve schedule (src.c:314-470: forward source steps, adjoint
+ * receiver steps and the checkpoint copies between them), which has
+ * numforw + nt steps (cqh_numforw). */
 int cq_batch_step(cq_batch b, int nsteps);
 /* Waits for the batch's queued work. */
 int cq_batch_sync(cq_batch b);
-/* Runs every shot to its nt and waits. */
+/* Runs every shot to its end and waits. */
 int cq_batch_run(cq_batch b);
-/* Copies shot `job`'s gather (nt*nr floats) to host memory. */
+/* Copies modeling shot `job`'s gather (nt*nr floats) to host memory. */
 int cq_batch_read_rec(cq_batch b, int job, float *rec);
 /* Copies shot `job`'s pressure field (nxx*nzz floats) to host memory. */
 int cq_batch_read_p(cq_batch b, int job, float *p);
-/* RTM batches: copies shot `job`'s raw cross-correlation image and source
+/* RTM shots: copies shot `job`'s raw cross-correlation image and source
  * illumination, nx*nz floats each over the interior (x-major), before the
- * Laplacian filter, mute and projection of sgpstd-qrtm/acps4.cu:909-945.
- * An RTM batch holds RTM or Q-RTM shots that share nt and snaps; cq_batch_run
- * drives the revolve schedule (src.c:314-470) for all of them in lock step. */
+ * Laplacian filter, mute and projection of sgpstd-qrtm/acps4.cu:909-945. */
 int cq_batch_read_image(cq_batch b, int job, float *img, float *ssg);
-/* Steps taken so far by the batch. */
+/* Global steps taken so far by the batch. */
 int cq_batch_time(cq_batch b);
 /* Device bytes held by the batch. */
 long long cq_batch_bytes(cq_batch b);
 int cq_batch_destroy(cq_batch b);
+
+/* Serving (continuous batching; no single counterpart in the reference, whose
+ * drivers loop over shots with MPI, sgpstd-qmodeling/acousticsgps4.c:255-452).
+ * A server keeps a device-resident batch.  Submitted jobs wait in a queue
+ * until the device memory budget and the job limit admit them, then join
+ * the launches of the jobs already running at the next global step.
+ * Finished jobs leave the batch, their results stay on the host until
+ * released. */
+typedef struct cq_server_s *cq_server;
+
+typedef struct {
+    int max_nxx, max_nzz;   /* largest padded window a job may have */
+    long long hbm_budget;   /* device bytes the resident jobs may hold; 0: 90% of free memory */
+    int max_jobs;           /* jobs resident at once; 0: no limit */
+} cq_server_desc;
+
+typedef enum { CQ_JOB_QUEUED = 0, CQ_JOB_RUNNING = 1, CQ_JOB_DONE = 2 } cq_job_state;
+
+int cq_server_open(cq_ctx ctx, const cq_server_desc *desc, cq_server *out);
+/* Validates and copies the job (all arrays); returns its id. */
+int cq_server_submit(cq_server s, const cq_job_desc *job, int *id);
+/* Admits queued jobs, enqueues up to nsteps global steps and retires the
+ * jobs that finish (their results are copied to the host); returns when the
+ * steps ran or no job is left.  Optional outputs: jobs running and queued. */
+int cq_server_advance(cq_server s, int nsteps, int *running, int *queued);
+int cq_server_poll(cq_server s, int id, int *state);
+/* Global steps at which the job was submitted, started and finished. */
+int cq_server_job_steps(cq_server s, int id, int *submitted, int *started, int *finished);
+/* A finished job's result: the gather (nt*nr) for modeling jobs, the raw
+ * image and illumination (nx*nz each, as cq_batch_read_image) for RTM jobs
+ * (ssg may be NULL for modeling jobs). */
+int cq_server_result(cq_server s, int id, float *out, float *ssg);
+int cq_server_release(cq_server s, int id);
+int cq_server_close(cq_server s);
 
 /* The plain path: one shot, one launch sequence per time step, the gather
  * row copied to the host and the stream synchronised after every step (the

@@ -86,8 +86,27 @@ def test_rtm_linear_in_data(ctx):
     assert engine.rel_l2(i2, 2.0 * i1) <= 1e-6
 
 
-def test_rtm_batch_rejects_mixed_schedules(ctx):
-    j1 = rtm_job(2, nt=60, snaps=4)
-    j2 = rtm_job(2, nt=60, snaps=5)
-    with pytest.raises(engine.CqError):
-        ctx.batch([j1, j2])
+def test_rtm_mixed_schedules_and_modeling_in_one_batch(ctx):
+    """Shots with different nt, checkpoint budgets and kinds (RTM, Q-RTM and a
+    forward-modeling shot) advance in the same launches, each on its own
+    revolve schedule: every result equals the shot run alone."""
+    jobs = [rtm_job(2, nt=60, snaps=4, seed=11), rtm_job(3, nt=90, snaps=7, seed=12),
+            rtm_job(2, nt=75, snaps=75 // 2, seed=13)]
+    fwd = synth.small_job(nxt=40, nzt=24, pml=5, nt=50, kind=1, seed=14)
+    alone = []
+    for j in jobs:
+        b = ctx.batch([j])
+        b.run()
+        alone.append(b.image(0))
+        b.close()
+    b = ctx.batch([fwd])
+    b.run()
+    rec_alone = b.rec(0)
+    b.close()
+    b = ctx.batch(jobs + [fwd])
+    b.run()
+    for i in range(len(jobs)):
+        img, ssg = b.image(i)
+        assert np.array_equal(img, alone[i][0]) and np.array_equal(ssg, alone[i][1]), i
+    assert np.array_equal(b.rec(len(jobs)), rec_alone)
+    b.close()

@@ -1,0 +1,102 @@
+"""Serving through the C ABI (cq_server_*): jobs submitted while others run
+join the same launches (continuous batching).  Each result equals the job
+run alone on a server of the same window limits (bit-exact: same tiles and
+plans), modeling gathers match the FP64 oracle (rel-L2 <= 1e-4), and the
+admission limits hold."""
+import numpy as np
+import pytest
+
+import orc
+from cqpy import engine, synth
+from test_gpu_rtm import rtm_job
+
+pytestmark = pytest.mark.gpu
+
+MAXX, MAXZ = 72, 48
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+def job_mix():
+    return [synth.small_job(nxt=40, nzt=24, pml=5, nt=120, kind=0, seed=21),
+            synth.small_job(nxt=41, nzt=25, pml=7, nt=100, kind=1, seed=22),
+            rtm_job(3, nt=80, snaps=5, seed=23),
+            synth.small_job(nxt=52, nzt=20, pml=6, nt=90, kind=0, seed=24),
+            rtm_job(2, nt=60, snaps=60, seed=25)]
+
+
+def same(a, b):
+    if isinstance(a, tuple):
+        return all(np.array_equal(x, y) for x, y in zip(a, b))
+    return np.array_equal(a, b)
+
+
+def run_alone(ctx, job):
+    s = ctx.server(MAXX, MAXZ)
+    i = s.submit(job)
+    while s.state(i) != engine.DONE:
+        s.advance(64)
+    out = s.result(i)
+    s.close()
+    return out
+
+
+def test_serve_staggered_equals_alone(ctx):
+    js = job_mix()
+    alone = [run_alone(ctx, j) for j in js]
+    s = ctx.server(MAXX, MAXZ)
+    ids = [s.submit(js[0])]
+    s.advance(30)
+    ids += [s.submit(js[1]), s.submit(js[2])]
+    s.advance(17)
+    ids += [s.submit(js[3]), s.submit(js[4])]
+    while any(s.state(i) != engine.DONE for i in ids):
+        s.advance(40)
+    for k, i in enumerate(ids):
+        print("job", k, "steps (submitted, started, finished)", s.steps(i), flush=True)
+        assert same(s.result(i), alone[k]), k
+    st = [s.steps(i) for i in ids]
+    # later jobs started while earlier ones were still running
+    assert st[1][1] < st[0][2] and st[3][1] < st[2][2]
+    for k in (0, 1, 3):
+        e = engine.rel_l2(s.result(ids[k]), orc.forward(js[k]))
+        print("job", k, "rel-L2 vs FP64 oracle", e)
+        assert e <= TOL
+    for i in ids:
+        s.release(i)
+    s.close()
+
+
+def test_serve_job_limit_runs_one_at_a_time(ctx):
+    js = job_mix()[:3]
+    s = ctx.server(MAXX, MAXZ, max_jobs=1)
+    ids = [s.submit(j) for j in js]
+    running, queued = s.advance(1)
+    assert running == 1 and queued == 2
+    while any(s.state(i) != engine.DONE for i in ids):
+        running, queued = s.advance(25)
+        assert running <= 1
+    st = [s.steps(i) for i in ids]
+    assert st[1][1] >= st[0][2] and st[2][1] >= st[1][2]
+    for k, i in enumerate(ids):
+        assert same(s.result(i), run_alone(ctx, js[k])), k
+    s.close()
+
+
+def test_serve_rejects_oversized_and_unfinished(ctx):
+    s = ctx.server(MAXX, MAXZ)
+    big = synth.small_job(nxt=80, nzt=24, pml=5, nt=10, kind=0, seed=26)
+    with pytest.raises(engine.CqError):
+        s.submit(big)
+    i = s.submit(job_mix()[0])
+    with pytest.raises(engine.CqError):
+        s.result(i)
+    with pytest.raises(engine.CqError):
+        s.release(i)
+    s.close()
