@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_copy_f4(const float4 *__restrict__ a, f
 struct cq_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::map<int, float2 *> twiddles;  // n -> device table w_n^j, j < n
+    std::map<std::vector<int>, float2 *> twiddles;  // radix sequence -> device twiddle table
     // open batches and servers; cq_close destroys what is left of them
     std::set<cq_batch_s *> batches;
     std::set<cq_server_s *> servers;
@@ -123,20 +123,29 @@ static int get_plan(cq_ctx ctx, int n, int need, CqPlan &p, const float2 *&tw)
     if (!make_plan(n, need, p))
         return set_err(CQ_EUNSUPPORTED,
                        "FFT length %d: no factorisation into radices 2..17 for %d points per workgroup", n, need);
-    auto itp = ctx->twiddles.find(n);
+    std::vector<int> key(p.rad, p.rad + p.ns);
+    auto itp = ctx->twiddles.find(key);
     if (itp != ctx->twiddles.end()) {
         tw = itp->second;
         return CQ_OK;
     }
-    std::vector<float2> h(n);
-    for (int j = 0; j < n; j++) {
-        const double a = -2.0 * M_PI * (double)j / (double)n;
-        h[j] = make_float2((float)cos(a), (float)sin(a));
+    // per-stage tables (cq_fft.h): stage s holds w_{Ns R}^(r k) at
+    // (r-1) Ns + k, r = 1..R-1, k < Ns; n - 1 entries in all (one spare)
+    std::vector<float2> h(n, make_float2(1.f, 0.f));
+    size_t o = 0;
+    for (int s = 0, Ns = 1; s < p.ns; Ns *= p.rad[s], s++) {
+        const int R = p.rad[s];
+        for (int r = 1; r < R; r++)
+            for (int k = 0; k < Ns; k++) {
+                const double a = -2.0 * M_PI * (double)r * (double)k / ((double)Ns * R);
+                h[o + (size_t)(r - 1) * Ns + k] = make_float2((float)cos(a), (float)sin(a));
+            }
+        o += (size_t)(R - 1) * Ns;
     }
     float2 *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(float2) * n));
     HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    ctx->twiddles[n] = d;
+    ctx->twiddles[key] = d;
     tw = d;
     return CQ_OK;
 }

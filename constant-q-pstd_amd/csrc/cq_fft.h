@@ -235,7 +235,9 @@ extern __shared__ float2 cq_lds[];
 // one Stockham stage over L lines in LDS
 // ---------------------------------------------------------------------------
 // bo   : offset of the first line in cq_lds; L lines LS slots apart
-// to   : offset of the twiddle table, tw[j] = w_n^j
+// to   : offset of this stage's twiddles, tw[(r-1) Ns + k] = w_{Ns R}^(r k)
+//        (r = 1..R-1, k < Ns), so lanes with consecutive k read consecutive
+//        slots (no bank conflicts)
 // Ns   : product of the radices of the previous stages
 // mh   : optional multiplier (global, length n) applied to the outputs
 template <int R, bool INV>
@@ -247,7 +249,6 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     constexpr int NB = cq_nb(R);
     const int m = n / R;
     const int total = L * m;
-    const int twmul = n / (Ns * R);
     // exact quotients by float reciprocal: operands are below 2^14, so
     // (i + 0.5) / d never lies within rounding distance of an integer
     const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
@@ -263,12 +264,11 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
             const int j = idx - l * m;
             const int k = j - (int)(((float)j + 0.5f) * inv_ns) * Ns;
             const float2 *src = buf + l * LS;
-            const int tstep = k * twmul;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 float2 x = src[cq_pad(j + r * m)];
-                if (r > 0 && tstep != 0) {
-                    const float2 w = tw[r * tstep];
+                if (r > 0 && k != 0) {
+                    const float2 w = tw[(r - 1) * Ns + k];
                     x = INV ? cmulc(x, w) : cmul(x, w);
                 }
                 v[b][r] = x;
@@ -313,16 +313,22 @@ __device__ __forceinline__ void cq_stage_rt(int R, int bo, int LS, int n, int L,
     }
 }
 
-// Forward transform of L lines at buf, twiddle table at tw (both inside
-// cq_lds), optional multiplier fused into the last stage.
+// Twiddle table of a plan (n - 1 entries, built on the host by the same rule):
+// the stages' tables one after another, stage s holding (R_s - 1) Ns entries
+// tw[(r-1) Ns + k] = w_{Ns R_s}^(r k) = exp(-2 pi i r k / (Ns R_s)).
+
+// Forward transform of L lines at buf, the plan's twiddle table at tw (both
+// inside cq_lds), optional multiplier fused into the last stage.
 __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,
                                            const float2 *tw, const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    const int bo = (int)(buf - cq_lds), to = (int)(tw - cq_lds);
+    const int bo = (int)(buf - cq_lds);
+    int to = (int)(tw - cq_lds);
     int Ns = 1;
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
         cq_stage_rt<false>(R, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
+        to += (R - 1) * Ns;
         Ns *= R;
     }
 }
@@ -330,11 +336,13 @@ __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS,
 __device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS, int L,
                                            const float2 *tw)
 {
-    const int bo = (int)(buf - cq_lds), to = (int)(tw - cq_lds);
+    const int bo = (int)(buf - cq_lds);
+    int to = (int)(tw - cq_lds);
     int Ns = 1;
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
         cq_stage_rt<true>(R, bo, LS, p.n, L, Ns, to, nullptr);
+        to += (R - 1) * Ns;
         Ns *= R;
     }
 }
