@@ -149,6 +149,36 @@ def serve_bench(rate, window_s, seed, chunk):
             "steps_per_advance": chunk}
 
 
+def qrtm_bench(nshots, nt, snaps):
+    """Q-RTM with revolve checkpointing on the 800 x 3000 workload (2100 x 880
+    padded windows, `snaps` checkpoints per shot), all shots in one batch, run
+    to completion with a shortened time axis (nt steps).  Executed
+    cell-steps count every forward and adjoint step the schedules take
+    (numforw + nt per shot); useful ones are 2 nt per shot."""
+    from cqpy import engine, host, synth
+    jobs = synth.qrtm_jobs(nshots=nshots, nt=nt, snaps=snaps)
+    cells = sum(j["nxx"] * j["nzz"] for j in jobs)
+    nf = host.numforw(nt, snaps)
+    ctx = engine.Context(0)
+    b = ctx.batch(jobs)
+    gb = b.nbytes() / 1e9
+    t0 = time.perf_counter()
+    b.run()
+    el = time.perf_counter() - t0
+    b.close()
+    ctx.close()
+    executed = cells * (nf + nt) / el / 1e6
+    return {"workload": "qrtm: %d Q-RTM shots, 800x3000 model, 2100x880 padded windows, 40-pt CPML, "
+                        "dx=6.25 m, dt=0.5 ms, %d checkpoints, nt=%d (time axis shortened from 6000)"
+                        % (nshots, snaps, nt),
+            "shots": nshots, "device_GB": round(gb, 2), "wall_s": round(el, 3),
+            "executed_steps_per_shot": nf + nt, "useful_steps_per_shot": 2 * nt,
+            "executed_mcell_steps_per_s": round(executed, 1),
+            "useful_mcell_steps_per_s": round(executed * 2 * nt / (nf + nt), 1),
+            "useful_mcell_steps_per_s_at_nt6000": round(
+                executed * 2 * 6000 / (host.numforw(6000, snaps) + 6000), 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +194,9 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a streaming copy of this many GB (counter calibration)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) phase")
+    ap.add_argument("--no-qrtm", action="store_true", help="skip the Q-RTM phase")
+    ap.add_argument("--qrtm-shots", type=int, default=32)
+    ap.add_argument("--qrtm-nt", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-window", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=50)
@@ -250,6 +283,12 @@ def main():
         serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk)
         print("serve:", json.dumps(serve), file=sys.stderr, flush=True)
 
+    qrtm = None
+    if not args.no_qrtm:
+        print("Q-RTM phase", file=sys.stderr, flush=True)
+        qrtm = qrtm_bench(args.qrtm_shots, args.qrtm_nt, 40)
+        print("qrtm:", json.dumps(qrtm), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -284,6 +323,7 @@ def main():
                      "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},
         "cpu_baseline": cpu,
         "serve": serve,
+        "qrtm": qrtm,
     }
     print(json.dumps(out))
 

@@ -266,3 +266,32 @@ def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 1216), nz_range
         t += rng.exponential(1.0 / rate)
         k += 1
     return out
+
+
+def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402):
+    """Q-RTM workload: shots over an 800 x 3000 model with a 40-point CPML,
+    dx = dz = 6.25 m, dt = 0.5 ms, 20 Hz Ricker, receivers +-1000 cells
+    around each source at the surface (window widened to 2020 columns so the
+    padded width 2100 = 2^2 3 5^2 7 is FFT-friendly), `snaps` checkpoints,
+    stabiliser 1e-20 on both wavefields.  The migration model is a layered
+    vp 1500-4500 m/s / Q 25-250 model smoothed with a Gaussian of 12 cells;
+    observed data are seeded random traces (the run time does not depend on
+    their content).  kind 3 = Q-RTM, 2 = acoustic RTM."""
+    nxt, nzt, dx, pml = 3000, 800, 6.25, 40
+    vp, rho, q = layered_model(nxt, nzt, dx, seed=seed, nlayers=8, vmin=1500.0, vmax=4500.0,
+                               qrange=(25.0, 250.0), smooth_sigma=12.0)
+    nx = fft_friendly(2001 + 2 * pml) - 2 * pml
+    first, last = 1000, nxt - 1001
+    rng = np.random.default_rng(obs_seed)
+    jobs = []
+    for i in range(nshots):
+        c = first + (last - first) * i // max(1, nshots - 1)
+        left = c - nx // 2
+        recs = np.arange(max(0, c - 1000), min(nxt, c + 1001))
+        job = make_window_job(kind, vp, rho, nxt, nzt, dx, dx, 0.0005, nt, 20.0, left, nx, c, 2,
+                              recs, 2, pml, qt=q, fo=20.0)
+        job["obs"] = rng.standard_normal((nt, len(job["gxg"]))).astype(np.float32)
+        job["snaps"] = snaps
+        job["etas"] = job["etar"] = 1e-20
+        jobs.append(job)
+    return jobs

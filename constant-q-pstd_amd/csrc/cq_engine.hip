@@ -417,6 +417,8 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
     }
 }
 
+static bool r1_fused_for(int nxx, int tk);
+
 // Allocates and uploads one shot for the batch's tile shapes.  With
 // bytes_only != null only its device footprint is computed.
 static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_only = nullptr)
@@ -447,10 +449,12 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.qsign = (j.kind == CQ_KIND_QRTM) ? -1.0f : 1.0f;
     base.stab = (j.kind == CQ_KIND_QRTM) ? 1 : 0;
     base.gather = rtm ? 0 : 1;
+    base.r1_fused = q && r1_fused_for(nxx, tk) ? 1 : 0;
     int rc;
-    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles up to
-    // 3 tk lines of nxx (k_q_r1); column tiles cw/2 lines of nzz
-    const int need_x = std::max((tz / 2) * nxx, q ? 3 * tk * nxx : 0), need_z = (cw / 2) * nzz;
+    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 3 tk
+    // (fused k_q_r1) or tk lines of nxx; column tiles cw/2 lines of nzz
+    const int need_x = std::max((tz / 2) * nxx, q ? (base.r1_fused ? 3 : 1) * tk * nxx : 0),
+              need_z = (cw / 2) * nzz;
     if ((rc = get_plan(ctx, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
     if ((rc = get_plan(ctx, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
 
@@ -707,14 +711,28 @@ static const int kTzCand[] = {16, 12, 8};
 static const int kCwCand[] = {32, 24, 16, 12, 8, 4, 2};
 static const int kTkCand[] = {8, 4, 2};
 
+// k_q_r1 inverts its three symbols together (3 tk lines) when a plan and the
+// LDS budget allow it, else one at a time (2 tk lines: spectrum + work).
+static bool r1_fused_for(int nxx, int tk)
+{
+    return has_plan(nxx, 3 * tk * nxx) && lds_lines(nxx, 3 * tk) <= kLdsTile;
+}
+static size_t r1_lds(int nxx, int tk) { return lds_lines(nxx, r1_fused_for(nxx, tk) ? 3 * tk : 2 * tk); }
+static bool r1_ok(int nxx, int tk, size_t lim)
+{
+    return (has_plan(nxx, 3 * tk * nxx) && lds_lines(nxx, 3 * tk) <= lim) ||
+           (has_plan(nxx, tk * nxx) && lds_lines(nxx, 2 * tk) <= lim);
+}
+
 // LDS of each pass for windows up to ax x az (acoustic) and qx x qz
-// (viscoacoustic).
-static void set_lds(cq_batch b, int ax, int az, int qx, int qz)
+// (viscoacoustic); k_q_r1 takes the largest need of the lengths in qxs.
+static void set_lds(cq_batch b, int ax, int az, int qx, int qz, const std::vector<int> &qxs)
 {
     b->lds[P_AROW] = lds_lines(ax, b->tz / 2);
     b->lds[P_ACOL] = lds_lines(az, b->cw / 2);
     b->lds[P_QC1] = lds_lines(qz, b->cwq / 2);
-    b->lds[P_QR1] = lds_lines(qx, 3 * b->tk);
+    b->lds[P_QR1] = 0;
+    for (int n : qxs) b->lds[P_QR1] = std::max(b->lds[P_QR1], r1_lds(n, b->tk));
     b->lds[P_QC2] = lds_lines(qz, b->cwq / 2);
     b->lds[P_QR2] = lds_lines(qx, b->tk);
     b->lds[P_QC3] = lds_lines(qz, b->cwq / 2);
@@ -734,8 +752,10 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
 {
     int iz = 0, ic = 0, icq = 0, ik = 0;   // indices into the candidate lists
     int ax = 0, az = 0, qx = 0, qz = 0;
+    std::vector<int> qxs;
     for (size_t i = 0; i < req.size(); i++) {
         const TileReq &j = req[i];
+        if (j.q) qxs.push_back(j.nxx);
         if (!has_plan(j.nxx, 1))
             return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", (int)i, j.nxx);
         if (!has_plan(j.nzz, 1))
@@ -747,9 +767,7 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
         auto col_bad = [&](int c, size_t lim) {
             return !has_plan(j.nzz, (c / 2) * j.nzz) || lds_lines(j.nzz, c / 2) > lim;
         };
-        auto krow_bad = [&](int k, size_t lim) {
-            return !has_plan(j.nxx, 3 * k * j.nxx) || lds_lines(j.nxx, 3 * k) > lim;
-        };
+        auto krow_bad = [&](int k, size_t lim) { return !r1_ok(j.nxx, k, lim); };
         int &ick = q ? icq : ic;
         while (iz < 2 && row_bad(kTzCand[iz], kLdsTile)) iz++;
         while (ick < 6 && col_bad(kCwCand[ick], kLdsTile)) ick++;
@@ -767,7 +785,7 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
     b->tk = kTkCand[ik];
     b->maxnxx = std::max(ax, qx);
     b->maxnzz = std::max(az, qz);
-    set_lds(b, ax, az, qx, qz);
+    set_lds(b, ax, az, qx, qz, qxs);
     return CQ_OK;
 }
 
@@ -1077,22 +1095,26 @@ struct cq_server_s {
 static int server_tiles(cq_batch b, int mx, int mz)
 {
     const int cap = cq_stage_capacity(16);
-    int iz = 0, ic = 0, icq = 0, ik = 0;
-    auto lim = [&](size_t lds, int pts) { return lds > kLdsTile || pts > cap; };
-    while (iz < 2 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx)) iz++;
-    while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz)) ic++;
-    icq = ic;
-    while (ik < 2 && lim(lds_lines(mx, 3 * kTkCand[ik]), 3 * kTkCand[ik] * mx)) ik++;
+    int iz = 0, ic = 0, ik = 0;
+    auto lim = [&](size_t lds, int pts, size_t budget) { return lds > budget || pts > cap; };
+    auto r1_bad = [&](int k, size_t budget) {
+        return lim(lds_lines(mx, 3 * k), 3 * k * mx, budget) && lim(lds_lines(mx, 2 * k), k * mx, budget);
+    };
+    while (iz < 2 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
+    while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz, kLdsTile)) ic++;
+    while (ik < 2 && r1_bad(kTkCand[ik], kLdsTile)) ik++;
     b->tz = kTzCand[iz];
     b->cw = kCwCand[ic];
-    b->cwq = kCwCand[icq];
+    b->cwq = kCwCand[ic];
     b->tk = kTkCand[ik];
-    if (lds_lines(mx, b->tz / 2) > kLdsMax || lds_lines(mz, b->cw / 2) > kLdsMax ||
-        lds_lines(mx, 3 * b->tk) > kLdsMax || (b->tz / 2) * mx > cap || 3 * b->tk * mx > cap)
+    if (lim(lds_lines(mx, b->tz / 2), (b->tz / 2) * mx, kLdsMax) || lim(lds_lines(mz, b->cw / 2), (b->cw / 2) * mz, kLdsMax) ||
+        r1_bad(b->tk, kLdsMax))
         return set_err(CQ_EUNSUPPORTED, "server windows up to %dx%d are too large for one workgroup", mx, mz);
     b->maxnxx = mx;
     b->maxnzz = mz;
-    set_lds(b, mx, mz, mx, mz);
+    set_lds(b, mx, mz, mx, mz, {});
+    // any job up to mx: fused k_q_r1 within the tile budget, or split
+    b->lds[P_QR1] = std::max(std::min(kLdsTile, lds_lines(mx, 3 * b->tk)), lds_lines(mx, 2 * b->tk));
     return CQ_OK;
 }
 

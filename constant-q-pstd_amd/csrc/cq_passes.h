@@ -337,6 +337,23 @@ __device__ __forceinline__ void krow_mult3(const JobG &J, float2 *buf, int k0)
     }
 }
 
+// Line dst + l (of TK) = spectrum line l x frac_symbol(which) / N.
+template <int TK>
+__device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst, int k0, int which)
+{
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
+        const float fkx = (float)ikx * J.dkx;
+        const float fkz = (float)(k0 + l) * J.dkz;
+        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
+        const float m = frac_symbol(which, k2) * J.invN;
+        const float2 v = buf[pe];
+        buf[dst * LS + pe] = make_float2(v.x * m, v.y * m);
+    }
+}
+
 // buf[l][kx] *= symbol(kx, k0 + l) / N.
 // which = 4: the stabilised compensation multiplier of Q-RTM,
 //   S = k - 2 e k (1 + tc dt k) / (exp(-tc tt k) + e exp(tc dt k)),  k = |k|^1.01,
@@ -571,14 +588,26 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
     cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
-    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566),
-    // inverse-transformed together
-    krow_mult3<TK>(J, buf, k0);
-    __syncthreads();
-    cq_fft_inv(J.px, buf, LS, 3 * TK, tw);
-    krow_store<TK>(buf, J, J.s1, k0);
-    krow_store<TK>(buf + TK * LS, J, J.s3, k0);
-    krow_store<TK>(buf + 2 * TK * LS, J, J.s4, k0);
+    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566)
+    if (J.r1_fused) {
+        // inverse-transformed together (3 TK lines)
+        krow_mult3<TK>(J, buf, k0);
+        __syncthreads();
+        cq_fft_inv(J.px, buf, LS, 3 * TK, tw);
+        krow_store<TK>(buf, J, J.s1, k0);
+        krow_store<TK>(buf + TK * LS, J, J.s3, k0);
+        krow_store<TK>(buf + 2 * TK * LS, J, J.s4, k0);
+    } else {
+        // one symbol at a time (long lines: 2 TK lines of LDS)
+        float2 CQ_GLOBAL *outs[3] = {J.s1, J.s3, J.s4};
+        for (int w = 0; w < 3; w++) {
+            krow_mult_to<TK>(J, buf, TK, k0, w);
+            __syncthreads();
+            cq_fft_inv(J.px, buf + TK * LS, LS, TK, tw);
+            krow_store<TK>(buf + TK * LS, J, outs[w], k0);
+            __syncthreads();
+        }
+    }
     __syncthreads();
     // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
     krow_fill<TK>(buf, J, J.s2, k0);

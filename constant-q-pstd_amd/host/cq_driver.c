@@ -196,6 +196,35 @@ int main(int argc, char **argv)
         }
         cq_batch bt;
         if (cq_batch_create(ctx, jd, ns, &bt)) DIE("cq_batch_create: %s", cq_last_error());
+        if (!rtm && P.flagout == 1) {
+            /* snapshots (sgpstd-qmodeling/acps4.cu:482-494, 624-640): the
+             * interior of p after every step t with t % 10 == 0, appended
+             * x-major to ../obsdata/snapa<shot>_<nz>_<nx>.bin (snapva for the
+             * viscoacoustic binary), each file truncated first */
+            float *pf = (float *)malloc(sizeof(float) * (size_t)nxx * nzz);
+            char name[512];
+            for (int k = 0; k < ns; k++) {
+                snprintf(name, sizeof name, "%s/%s%d_%d_%d.bin", outdir, q ? "snapva" : "snapa", sb[k].shotid, nz, nx);
+                FILE *fs = fopen(name, "wb");
+                if (!fs) DIE("cannot write %s", name);
+                fclose(fs);
+            }
+            for (int t = 0; t < tmax; t++) {
+                if (cq_batch_step(bt, 1)) DIE("cq_batch_step: %s", cq_last_error());
+                if (t % 10) continue;
+                for (int k = 0; k < ns; k++) {
+                    if (cq_batch_read_p(bt, k, pf)) DIE("cq_batch_read_p: %s", cq_last_error());
+                    snprintf(name, sizeof name, "%s/%s%d_%d_%d.bin", outdir, q ? "snapva" : "snapa", sb[k].shotid, nz, nx);
+                    FILE *fs = fopen(name, "ab");
+                    if (!fs) DIE("cannot write %s", name);
+                    for (int ix = pml; ix < nxx - pml; ix++)
+                        if (fwrite(pf + (size_t)ix * nzz + pml, sizeof(float), (size_t)nz, fs) != (size_t)nz)
+                            DIE("short write of %s", name);
+                    fclose(fs);
+                }
+            }
+            free(pf);
+        }
         if (cq_batch_run(bt)) DIE("cq_batch_run: %s", cq_last_error());
         for (int k = 0; k < ns; k++) {
             shot_buf *b = &sb[k];

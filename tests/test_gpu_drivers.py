@@ -1,0 +1,119 @@
+"""The drop-in drivers end to end on a small generated dataset: reference-format
+parameter files, geometry files and float32 models in, reference-named
+outputs out.  aps / vaps gathers match the FP64 oracle on the same shots
+(rel-L2 <= 1e-4), snapshots (flagout = 1) have the reference's layout and
+equal the engine's pressure field, and artm / qrtm write [image |
+illumination] stacks of the model size."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from cqpy import engine, synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "constant-q-pstd_amd", "bin")
+GOLD = os.path.join(ROOT, "tests", "golden", "input")
+NXT, NZT, DX, NT, DT_MS, FD, FO, PML = 64, 32, 10.0, 100, 1.0, 30.0, 60.0, 10
+SHOTS = [20, 32, 44]
+HALF = 15                    # receivers at s-15 .. s+15 (cells)
+
+
+def param_file(path, golden, values):
+    labels = [l.rstrip("\n") for l in open(os.path.join(GOLD, golden))][0::2]
+    assert len(labels) == len(values), (golden, len(labels), len(values))
+    with open(path, "w") as f:
+        for lab, v in zip(labels, values):
+            f.write("%s\n  %s\n" % (lab, v))
+        f.write("--CPML width (grid points)\n  %d\n" % PML)
+
+
+@pytest.fixture(scope="module")
+def tree(tmp_path_factory):
+    t = tmp_path_factory.mktemp("cq")
+    for d in ("work", "input", "geometry", "output", "obsdata"):
+        (t / d).mkdir()
+    vp, rho, q = synth.layered_model(NXT, NZT, DX, seed=77, nlayers=3, vmin=1500.0, vmax=3000.0,
+                                     qrange=(30.0, 120.0))
+    vp.tofile(t / "output" / "vp.bin")
+    rho.tofile(t / "output" / "rho.bin")
+    q.tofile(t / "output" / "q.bin")
+    np.full(len(SHOTS), 2 * HALF + 1, np.int32).tofile(t / "geometry" / "tracenum.bin")
+    np.array([s * DX for s in SHOTS], np.float32).tofile(t / "geometry" / "shotxp.bin")
+    np.full(len(SHOTS), 2 * DX, np.float32).tofile(t / "geometry" / "shotzp.bin")
+    for i, s in enumerate(SHOTS):
+        (np.arange(s - HALF, s + HALF + 1) * DX).astype(np.float32).tofile(t / "geometry" / ("gxp%d.bin" % i))
+        np.full(2 * HALF + 1, 2 * DX, np.float32).tofile(t / "geometry" / ("gzp%d.bin" % i))
+    geo = ["../geometry/tracenum.bin", "../geometry/shotxp.bin", "../geometry/shotzp.bin",
+           "../geometry/gxp", "../geometry/gzp"]
+    head = [len(SHOTS), NT, DT_MS, FD]
+    grid = [NXT, NZT, DX, DX, 0.0]
+    param_file(t / "input" / "aps.txt", "ac2drealmodeling.txt",
+               head + grid + geo + ["../output/vp.bin", "../output/rho.bin", 1, 2])
+    param_file(t / "input" / "vaps.txt", "visac2drealmodeling.txt",
+               head + [FO] + grid + geo + ["../output/vp.bin", "../output/rho.bin", "../output/q.bin", 1, 2])
+    param_file(t / "input" / "artm.txt", "ac2drtm.txt",
+               head + grid + [1500.0] + geo + ["../output/vp.bin", "../output/rho.bin", "../obsdata/reca", 2, 12])
+    param_file(t / "input" / "qrtm.txt", "visac2drtm.txt",
+               head + [FO] + grid + [1500.0, 1e-20, 1e-20] + geo +
+               ["../output/vp.bin", "../output/rho.bin", "../output/q.bin", "../obsdata/recva", 2, 12])
+    return t, vp, rho, q
+
+
+def run(t, mode, batch=2):
+    print("driver", mode, flush=True)
+    r = subprocess.run([os.path.join(BIN, mode), "../input/%s.txt" % mode, "../obsdata", str(batch)],
+                       cwd=t / "work", capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def shot_job(kind, vp, rho, q, s):
+    gxp = (np.arange(s - HALF, s + HALF + 1) * DX).astype(np.float32)
+    gzp = np.full(2 * HALF + 1, 2 * DX, np.float32)
+    return synth.make_job(kind, vp, rho, NXT, NZT, DX, DX, DT_MS / 1000.0, NT, FD, s * DX, 2 * DX, gxp, gzp,
+                          PML, qt=q, fo=FO)
+
+
+@pytest.mark.parametrize("mode,kind,prefix", [("aps", 0, "reca"), ("vaps", 1, "recva")])
+def test_modeling_driver_vs_oracle(tree, mode, kind, prefix):
+    t, vp, rho, q = tree
+    run(t, mode)
+    nr = 2 * HALF + 1
+    for i, s in enumerate(SHOTS):
+        got = np.fromfile(t / "obsdata" / ("%s%d_%d.bin" % (prefix, i, nr)), np.float32).reshape(NT, nr)
+        job = shot_job(kind, vp, rho, q, s)
+        e = engine.rel_l2(got, orc.forward(job))
+        print(mode, "shot", i, "rel-L2 vs FP64 oracle", e)
+        assert e <= 1e-4
+    # snapshots: interior of p after every 10th step, x-major, appended
+    job = shot_job(kind, vp, rho, q, SHOTS[0])
+    nx, nz = job["nxx"] - 2 * PML, job["nzz"] - 2 * PML
+    snap = np.fromfile(t / "obsdata" / ("%s%d_%d_%d.bin" % ("snapa" if kind == 0 else "snapva", 0, nz, nx)),
+                       np.float32)
+    assert snap.size == (NT // 10) * nx * nz
+    with engine.Context(0) as ctx:
+        b = ctx.batch([job])
+        b.step(NT - 9)   # through t = 90
+        p = b.p(0).reshape(job["nxx"], job["nzz"])[PML:-PML, PML:-PML]
+        b.close()
+    last = snap.reshape(NT // 10, nx, nz)[-1]
+    e = engine.rel_l2(last, p)
+    print(mode, "last snapshot vs engine p", e)
+    assert e <= 1e-5
+
+
+@pytest.mark.parametrize("mode,out", [("artm", "RTM.BIN"), ("qrtm", "QRTM.BIN")])
+def test_rtm_driver_writes_stack(tree, mode, out):
+    t, vp, rho, q = tree
+    run(t, "aps" if mode == "artm" else "vaps")   # observed data
+    run(t, mode)
+    img = np.fromfile(t / "work" / out, np.float32)
+    assert img.size == 2 * NXT * NZT
+    img, ssg = img[:NXT * NZT], img[NXT * NZT:]
+    assert np.isfinite(img).all() and np.isfinite(ssg).all()
+    assert np.abs(img).max() > 0 and ssg.min() >= 0 and ssg.max() > 0

@@ -110,3 +110,20 @@ def test_rtm_mixed_schedules_and_modeling_in_one_batch(ctx):
         assert np.array_equal(img, alone[i][0]) and np.array_equal(ssg, alone[i][1]), i
     assert np.array_equal(b.rec(len(jobs)), rec_alone)
     b.close()
+
+
+def test_qrtm_long_lines_vs_oracle(ctx):
+    """Q-RTM on a 1575-column window (one-at-a-time symbol inverses in the
+    kz-row pass, stabilised multiplier on both wavefields)."""
+    job = rtm_job(3, nxt=1555, nzt=30, pml=10, nt=40, snaps=8, seed=17)
+    assert job["nxx"] == 1575
+    print("case long lines Q-RTM", job["nxx"], job["nzz"], flush=True)
+    trace, info = host.revolve_trace(40, 8)
+    ref_img, ref_ssg = orc.rtm(job, trace)
+    b = ctx.batch([job])
+    b.run()
+    img, ssg = b.image(0)
+    b.close()
+    ei, es = engine.rel_l2(img, ref_img), engine.rel_l2(ssg, ref_ssg)
+    print("rel-L2 image", ei, "illumination", es)
+    assert es <= TOL_SSG and ei <= TOL_IMG

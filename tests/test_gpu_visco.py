@@ -106,3 +106,18 @@ def test_config2_full_vs_oracle(ctx):
     e = engine.rel_l2(got, ref)
     print("config2 rel-L2 vs FP64 oracle:", e)
     assert e <= TOL
+
+
+def test_visco_long_lines_vs_oracle(ctx):
+    """A 1575-column window: the kz-row pass no longer fits its three symbol
+    inverses in one workgroup's LDS budget and inverts them one at a time."""
+    job = synth.small_job(nxt=1555, nzt=40, pml=10, nt=40, kind=1, seed=31)
+    assert job["nxx"] == 1575
+    print("case long lines", job["nxx"], job["nzz"], flush=True)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
