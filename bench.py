@@ -3,21 +3,26 @@
 Workload (default, "batch128"): the batched multi-shot forward-modeling job --
 128 shots on a 600 x 2400 model with a 40-point CPML, dx = dz = 5 m,
 dt = 0.4 ms, 20 Hz Ricker, shots every 18 cells, windows cropped to +-600
-cells around each source (uneven sizes).  One "step" advances every shot of
-the batch by one time step (two fused HIP passes per shot tile).  All inputs
-are resident in HBM before the timed region.
+cells around each source (uneven sizes), half acoustic and half
+viscoacoustic.  One "step" advances every shot of the batch by one time step
+(2 fused passes per acoustic shot, 6 per viscoacoustic shot).  All inputs are
+resident in HBM before the timed region.
 
 Reported: batched Mcell-steps/s (the metric), the plain one-shot-per-launch
-path on one shot of the same workload, the row pass's HBM roofline fraction
-(HIP-event timing on the engine's stream), and the FP32 reference-faithful CPU
-path (the oracle's FP32 build: 9 full 2-D C2C FFTs per step, OpenMP) timed on
-a bounded sample of the same workload.
+path on one shot of the same workload, the dominant pass's HBM roofline
+fraction (HIP-event timing on the engine's stream) with its measured traffic,
+the FP32 reference-faithful CPU path (the oracle's FP32 build: 9 full 2-D C2C
+FFTs per step, OpenMP) timed on a bounded sample of the same workload, the
+serving phase (continuous batching of Poisson arrivals: throughput, p50/p99
+latency) and the Q-RTM phase (32 revolve-checkpointed shots).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch128|config1]
+                       [--no-serve] [--no-qrtm] [--no-plain] [--no-cpu]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -84,6 +89,27 @@ def cpu_baseline(job, steps, threads):
     orc.forward(j, bits=32)
     dt = time.perf_counter() - t0
     return j["nxx"] * j["nzz"] * steps / dt / 1e6, dt
+
+
+PROFILE = os.path.join(ROOT, "profiles", "batch128_mixed_passes.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, each
+    calibrated with a streaming copy) from the committed rocprofv3 summary of
+    the default workload, or None."""
+    try:
+        text = open(PROFILE).read().split("## HBM bytes per dispatch", 1)
+    except OSError:
+        return None
+    if len(text) < 2:
+        return None
+    for line in text[1].splitlines():
+        # | kernel<tile> | FETCH KiB | WRITE KiB | HBM bytes | algorithmic bytes | ratio |
+        m = re.match(r"\| (k_\w+)<\d+> \| [\d.]+ \| [\d.]+ \| ([0-9.e+]+) \|", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(2))
+    return None
 
 
 def serve_bench(rate, window_s, seed, chunk):
@@ -234,6 +260,7 @@ def main():
         wl = {"workload": "config1: one acoustic shot, 459x220 padded, dt=0.5 ms",
               "shots": 1, "kinds": "acoustic", "cells_per_step": jobs[0]["nxx"] * jobs[0]["nzz"]}
     cells = wl["cells_per_step"]
+    default_workload = args.workload == "batch128" and args.kinds == "mixed" and args.shots == 128
     ctx = engine.Context(0)
     b = ctx.batch(jobs)
     print("batch built: %d shots, %.2f GB on device" % (len(jobs), b.nbytes() / 1e9),
@@ -317,7 +344,8 @@ def main():
         "plain_mcell_steps_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom) if default_workload else None,
                      "kernel": dom, "passes": passes,
                      "step_bytes": step_bytes,
                      "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},

@@ -300,6 +300,7 @@ struct JobH {
     float src0 = 0.f;       // rik[0] * dt * 1000 (checkpoint-0 quirk)
     int t0 = 0, nsched = 0; // global steps t0 .. t0 + nsched - 1
     std::vector<CkptOp> ops;  // RTM checkpoint copies, by local step
+    std::vector<int4> hsched; // RTM step records (host copy of d[r].sched)
     size_t op_next = 0;
 };
 
@@ -318,9 +319,9 @@ struct cq_batch_s {
     // job table: entry i = role 0 of jobs[i], entry cap + i = role 1
     JobD *djobs = nullptr;
     size_t djobs_cap = 0;                  // entries per role the table holds
-    int2 *ditems[P_COUNT] = {};            // (table entry, tile origin), XCD-ordered
-    size_t ditems_cap[P_COUNT] = {};
-    int nitems[P_COUNT] = {};
+    int2 *ditems[2][P_COUNT] = {};         // per role: (table entry, tile origin), XCD-ordered
+    size_t ditems_cap[2][P_COUNT] = {};
+    int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
     int maxnxx = 0, maxnzz = 0;
@@ -332,7 +333,8 @@ struct cq_batch_s {
     {
         for (auto &j : jobs) (void)hipFree(j.mem);
         (void)hipFree(djobs);
-        for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[i]);
+        for (int r = 0; r < 2; r++)
+            for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
@@ -497,6 +499,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         o_sched = A.take<int4>(hsched.size());
     }
     h.nsched = rtm ? (int)hsched.size() : j.nt;
+    h.hsched = hsched;
     h.t0 = 0;
     h.op_next = 0;
     const size_t total = A.take<char>(0);
@@ -707,7 +710,7 @@ static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_
 // line), columns per column tile (two per packed line), kz rows per kz-row
 // tile.  Widths that are not powers of two fill a workgroup's 512 threads
 // better when a power of two does not fit the LDS budget.
-static const int kTzCand[] = {16, 12, 8};
+static const int kTzCand[] = {16, 12, 8, 4};
 static const int kCwCand[] = {32, 24, 16, 12, 8, 4, 2};
 static const int kTkCand[] = {8, 4, 2};
 
@@ -769,7 +772,7 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
         };
         auto krow_bad = [&](int k, size_t lim) { return !r1_ok(j.nxx, k, lim); };
         int &ick = q ? icq : ic;
-        while (iz < 2 && row_bad(kTzCand[iz], kLdsTile)) iz++;
+        while (iz < 3 && row_bad(kTzCand[iz], kLdsTile)) iz++;
         while (ick < 6 && col_bad(kCwCand[ick], kLdsTile)) ick++;
         if (q)
             while (ik < 2 && krow_bad(kTkCand[ik], kLdsTile)) ik++;
@@ -807,7 +810,7 @@ static int rebuild_tables(cq_batch b)
     const int cap = (int)b->djobs_cap;
     std::vector<JobD> hd(2 * (size_t)cap);
     memset(hd.data(), 0, sizeof(JobD) * hd.size());
-    std::vector<int2> items[P_COUNT];
+    std::vector<int2> items[2][P_COUNT];
     b->tmax = 0;
     for (size_t i = 0; i < n; i++) {
         const JobH &h = b->jobs[i];
@@ -816,38 +819,41 @@ static int rebuild_tables(cq_batch b)
         b->tmax = std::max(b->tmax, h.t0 + h.nsched);
         for (int r = 0; r < h.nroles; r++) {
             const int e = r ? cap + (int)i : (int)i;
+            std::vector<int2> *it = items[r];
             if (!is_q(h.kind)) {
-                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) items[P_AROW].push_back(make_int2(e, z0));
-                for (int x0 = 0; x0 < h.nxx; x0 += b->cw) items[P_ACOL].push_back(make_int2(e, x0));
+                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_AROW].push_back(make_int2(e, z0));
+                for (int x0 = 0; x0 < h.nxx; x0 += b->cw) it[P_ACOL].push_back(make_int2(e, x0));
             } else {
                 for (int x0 = 0; x0 < h.nxx; x0 += b->cwq) {
-                    items[P_QC1].push_back(make_int2(e, x0));
-                    items[P_QC2].push_back(make_int2(e, x0));
-                    items[P_QC3].push_back(make_int2(e, x0));
+                    it[P_QC1].push_back(make_int2(e, x0));
+                    it[P_QC2].push_back(make_int2(e, x0));
+                    it[P_QC3].push_back(make_int2(e, x0));
                 }
                 for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) {
-                    items[P_QR1].push_back(make_int2(e, k0));
-                    items[P_QR2].push_back(make_int2(e, k0));
+                    it[P_QR1].push_back(make_int2(e, k0));
+                    it[P_QR2].push_back(make_int2(e, k0));
                 }
-                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) items[P_QR3].push_back(make_int2(e, z0));
+                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_QR3].push_back(make_int2(e, z0));
             }
         }
     }
-    std::vector<int2> ord[P_COUNT];
+    std::vector<int2> ord[2][P_COUNT];
     HIPCHK(hipMemcpyAsync(b->djobs, hd.data(), sizeof(JobD) * hd.size(), hipMemcpyHostToDevice, s));
-    for (int p = 0; p < P_COUNT; p++) {
-        b->nitems[p] = 0;
-        if (items[p].empty()) continue;
-        ord[p] = xcd_order(items[p]);
-        if (ord[p].size() > b->ditems_cap[p]) {
-            (void)hipFree(b->ditems[p]);
-            b->ditems[p] = nullptr;
-            b->ditems_cap[p] = std::max(ord[p].size(), 2 * b->ditems_cap[p]);
-            HIPCHK(hipMalloc(&b->ditems[p], sizeof(int2) * b->ditems_cap[p]));
+    for (int r = 0; r < 2; r++)
+        for (int p = 0; p < P_COUNT; p++) {
+            b->nitems[r][p] = 0;
+            if (items[r][p].empty()) continue;
+            ord[r][p] = xcd_order(items[r][p]);
+            if (ord[r][p].size() > b->ditems_cap[r][p]) {
+                (void)hipFree(b->ditems[r][p]);
+                b->ditems[r][p] = nullptr;
+                b->ditems_cap[r][p] = std::max(ord[r][p].size(), 2 * b->ditems_cap[r][p]);
+                HIPCHK(hipMalloc(&b->ditems[r][p], sizeof(int2) * b->ditems_cap[r][p]));
+            }
+            HIPCHK(hipMemcpyAsync(b->ditems[r][p], ord[r][p].data(), sizeof(int2) * ord[r][p].size(),
+                                  hipMemcpyHostToDevice, s));
+            b->nitems[r][p] = (int)ord[r][p].size();
         }
-        HIPCHK(hipMemcpyAsync(b->ditems[p], ord[p].data(), sizeof(int2) * ord[p].size(), hipMemcpyHostToDevice, s));
-        b->nitems[p] = (int)ord[p].size();
-    }
     // the host vectors must outlive the copies
     HIPCHK(hipStreamSynchronize(s));
     return CQ_OK;
@@ -880,21 +886,22 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
 }
 
 template <typename K>
-static int launch(cq_batch b, K kernel, int pass, int step)
+static int launch(cq_batch b, K kernel, int pass, int role, int step)
 {
     const size_t lds = b->lds[pass];
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kernel, dim3(b->nitems[pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs,
-                       b->ditems[pass], step);
+    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs,
+                       b->ditems[role][pass], step);
     HIPCHK(hipGetLastError());
     return CQ_OK;
 }
 
-#define CQ_ARGS pass, step
+#define CQ_ARGS pass, role, step
 #define CQ_DISPATCH_TZ(K)                                                          \
     (b->tz == 16   ? launch(b, K<16>, CQ_ARGS)                                     \
-     : b->tz == 12 ? launch(b, K<12>, CQ_ARGS) : launch(b, K<8>, CQ_ARGS))
+     : b->tz == 12 ? launch(b, K<12>, CQ_ARGS)                                     \
+     : b->tz == 8  ? launch(b, K<8>, CQ_ARGS) : launch(b, K<4>, CQ_ARGS))
 #define CQ_DISPATCH_CW(K, W)                                                       \
     (W == 32   ? launch(b, K<32>, CQ_ARGS)                                         \
      : W == 24 ? launch(b, K<24>, CQ_ARGS)                                         \
@@ -907,7 +914,7 @@ static int launch(cq_batch b, K kernel, int pass, int step)
     (b->tk == 8 ? launch(b, K<8>, CQ_ARGS)                                         \
      : b->tk == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
 
-static int launch_pass(cq_batch b, int pass, int step)
+static int launch_pass(cq_batch b, int pass, int role, int step)
 {
     switch (pass) {
     case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
@@ -922,9 +929,18 @@ static int launch_pass(cq_batch b, int pass, int step)
     return set_err(CQ_EINVAL, "unknown pass %d", pass);
 }
 
-// Global step `step` of every wavefield of the batch (one launch per pass).
+// Global step `step` of every wavefield of the batch: one launch per pass
+// and role, for the roles that advance at this step (a modeling shot's one
+// wavefield while it runs; an RTM shot's source or receiver wavefield as its
+// schedule says).
 static int one_step(cq_batch b, int step)
 {
+    bool active[2] = {false, false};
+    for (const JobH &h : b->jobs) {
+        const int s = step - h.t0;
+        if (s < 0 || s >= h.nsched) continue;
+        active[h.hsched.empty() ? 0 : h.hsched[s].x] = true;
+    }
     if (b->timing) {
         hipEvent_t e0;
         HIPCHK(hipEventCreate(&e0));
@@ -932,16 +948,19 @@ static int one_step(cq_batch b, int step)
         b->ev.push_back(e0);
         b->ev_pass.push_back(-1);
     }
-    for (int p = 0; p < P_COUNT; p++) {
-        if (b->nitems[p] == 0) continue;
-        int rc = launch_pass(b, p, step);
-        if (rc) return rc;
-        if (b->timing) {
-            hipEvent_t e1;
-            HIPCHK(hipEventCreate(&e1));
-            HIPCHK(hipEventRecord(e1, b->ctx->stream));
-            b->ev.push_back(e1);
-            b->ev_pass.push_back(p);
+    for (int r = 0; r < 2; r++) {
+        if (!active[r]) continue;
+        for (int p = 0; p < P_COUNT; p++) {
+            if (b->nitems[r][p] == 0) continue;
+            int rc = launch_pass(b, p, r, step);
+            if (rc) return rc;
+            if (b->timing) {
+                hipEvent_t e1;
+                HIPCHK(hipEventCreate(&e1));
+                HIPCHK(hipEventRecord(e1, b->ctx->stream));
+                b->ev.push_back(e1);
+                b->ev_pass.push_back(p);
+            }
         }
     }
     return CQ_OK;
@@ -1100,7 +1119,7 @@ static int server_tiles(cq_batch b, int mx, int mz)
     auto r1_bad = [&](int k, size_t budget) {
         return lim(lds_lines(mx, 3 * k), 3 * k * mx, budget) && lim(lds_lines(mx, 2 * k), k * mx, budget);
     };
-    while (iz < 2 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
+    while (iz < 3 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
     while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz, kLdsTile)) ic++;
     while (ik < 2 && r1_bad(kTkCand[ik], kLdsTile)) ik++;
     b->tz = kTzCand[iz];

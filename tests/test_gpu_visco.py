@@ -121,3 +121,19 @@ def test_visco_long_lines_vs_oracle(ctx):
     e = engine.rel_l2(got, orc.forward(job))
     print("rel-L2 gather", e)
     assert e <= TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_very_long_lines_vs_oracle(ctx, kind):
+    """A 2025-column window: row tiles of 4 depths (two packed lines) fit the
+    LDS budget where 8 do not."""
+    job = synth.small_job(nxt=2005, nzt=24, pml=10, nt=30, kind=kind, seed=33)
+    assert job["nxx"] == 2025
+    print("case very long lines kind", kind, job["nxx"], job["nzz"], flush=True)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
