@@ -192,6 +192,12 @@ def qrtm_bench(nshots, nt, snaps):
     b.run()
     el = time.perf_counter() - t0
     b.close()
+    # per-pass device times over the first 300 global steps of a second run
+    b = ctx.batch(jobs)
+    b.enable_timing(True)
+    b.step(300)
+    kt = {k: round(v[0], 4) for k, v in b.kernel_times().items()}
+    b.close()
     ctx.close()
     executed = cells * (nf + nt) / el / 1e6
     return {"workload": "qrtm: %d Q-RTM shots, 800x3000 model, 2100x880 padded windows, 40-pt CPML, "
@@ -202,7 +208,8 @@ def qrtm_bench(nshots, nt, snaps):
             "executed_mcell_steps_per_s": round(executed, 1),
             "useful_mcell_steps_per_s": round(executed * 2 * nt / (nf + nt), 1),
             "useful_mcell_steps_per_s_at_nt6000": round(
-                executed * 2 * 6000 / (host.numforw(6000, snaps) + 6000), 1)}
+                executed * 2 * 6000 / (host.numforw(6000, snaps) + 6000), 1),
+            "pass_ms_first_300_steps": kt}
 
 
 def main():

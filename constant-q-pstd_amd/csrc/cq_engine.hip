@@ -296,7 +296,7 @@ struct JobH {
     size_t state_bytes = 0;
     int snaps = 0;
     int nt = 0, nr = 0, nxx = 0, nzz = 0, ldz = 0, kind = 0, nx = 0, nz = 0;
-    size_t src_cell = 0;    // xsg * ldz + zsg
+    size_t src_cell = 0;    // field index of (xsg, zsg)
     float src0 = 0.f;       // rik[0] * dt * 1000 (checkpoint-0 quirk)
     int t0 = 0, nsched = 0; // global steps t0 .. t0 + nsched - 1
     std::vector<CkptOp> ops;  // RTM checkpoint copies, by local step
@@ -486,8 +486,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     const int ntile_inj = q ? ntile_x : ntile_z;
     if (rtm) {
         o_obs = A.take<float>((size_t)j.nt * std::max(j.nr, 1));
-        o_img = A.take<float>((size_t)nx * nz);
-        o_ssg = A.take<float>((size_t)nx * nz);
+        o_img = A.take<float>(N);   // field layout, interior cells used
+        o_ssg = A.take<float>(N);
         o_inj = A.take<int>((size_t)3 * std::max(j.nr, 1));
         o_ioff = A.take<int>((size_t)ntile_inj + 1);
     }
@@ -517,7 +517,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     h.snaps = snaps;
     h.ckpt = mem + o_ckpt;
     h.nt = j.nt; h.nr = j.nr; h.nxx = nxx; h.nzz = nzz; h.ldz = ldz; h.kind = j.kind; h.nx = nx; h.nz = nz;
-    h.src_cell = (size_t)j.xsg * ldz + j.zsg;
+    h.src_cell = cq_q4(nxx, j.xsg, j.zsg >> 2) + (j.zsg & 3);
     h.src0 = j.rik[0] * j.dt * 1000.0f;
     HIPCHK(hipMemsetAsync(mem, 0, total, ctx->stream));
 
@@ -575,7 +575,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         }
     }
 
-    // per-cell coefficients (ldz-strided, zero padding)
+    // per-cell coefficients (field layout, zero padding)
     const float dt = j.dt;
     std::vector<float> hbx(N, 0.f), hbz(N, 0.f), hf2(N, 0.f), hf3(N, 0.f);
     std::vector<float> hq[4];
@@ -583,7 +583,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         for (int i = 0; i < 4; i++) hq[i].assign(N, 0.f);
     for (int ix = 0; ix < nxx; ix++)
         for (int iz = 0; iz < nzz; iz++) {
-            const size_t a = (size_t)ix * nzz + iz, g = (size_t)ix * ldz + iz;
+            const size_t a = (size_t)ix * nzz + iz, g = cq_q4(nxx, ix, iz >> 2) + (iz & 3);
             const float v = j.vp[a];
             hbx[g] = dt * j.bbx[a];
             hbz[g] = dt * j.bbz[a];
@@ -1034,13 +1034,28 @@ extern "C" int cq_batch_run(cq_batch b)
     return cq_batch_sync(b);
 }
 
+// Copies a field (device field layout) to host memory as f[ix * nzz + iz]
+// over ix in [x0, x0 + nx), iz in [z0, z0 + nz) (x-major, z fastest).
+static int read_field(const JobH &h, const float *dev, int x0, int nx, int z0, int nz, float *out)
+{
+    std::vector<float> f((size_t)h.nxx * h.ldz);
+    HIPCHK(hipMemcpy(f.data(), dev, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            const int gx = x0 + ix, gz = z0 + iz;
+            out[(size_t)ix * nz + iz] = f[cq_q4(h.nxx, gx, gz >> 2) + (gz & 3)];
+        }
+    return CQ_OK;
+}
+
 static int read_image(cq_ctx ctx, const JobH &h, float *img, float *ssg)
 {
     if (!is_rtm(h.kind)) return set_err(CQ_ESTATE, "not an RTM shot");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(img, h.d[0].img, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ssg, h.d[0].ssg, sizeof(float) * (size_t)h.nx * h.nz, hipMemcpyDeviceToHost));
-    return CQ_OK;
+    const int pml = (h.nxx - h.nx) / 2;
+    int rc = read_field(h, h.d[0].img, pml, h.nx, pml, h.nz, img);
+    if (rc == CQ_OK) rc = read_field(h, h.d[0].ssg, pml, h.nx, pml, h.nz, ssg);
+    return rc;
 }
 
 static int read_rec(cq_ctx ctx, const JobH &h, float *rec)
@@ -1069,9 +1084,7 @@ extern "C" int cq_batch_read_p(cq_batch b, int job, float *p)
     if (!b || job < 0 || job >= (int)b->jobs.size() || !p) return set_err(CQ_EINVAL, "cq_batch_read_p: bad arguments");
     const JobH &h = b->jobs[job];
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
-    HIPCHK(hipMemcpy2D(p, sizeof(float) * h.nzz, h.d[0].p, sizeof(float) * h.ldz, sizeof(float) * h.nzz, h.nxx,
-                       hipMemcpyDeviceToHost));
-    return CQ_OK;
+    return read_field(h, h.d[0].p, 0, h.nxx, 0, h.nzz, p);
 }
 
 extern "C" int cq_batch_time(cq_batch b) { return b ? b->t : -1; }

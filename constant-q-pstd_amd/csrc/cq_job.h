@@ -5,9 +5,14 @@
 #pragma once
 #include "cq_fft.h"
 
-// Layout of every real field: x-major, z fastest, leading dimension ldz
-// (nzz rounded up to 32 floats = 128 B, padding rows kept at zero).
+// Layout of every real field: depth quads major.  Cell (ix, iz) is float
+// (iz & 3) of the 16-byte quad (ix, iz >> 2) at cq_q4(nxx, ix, iz >> 2):
+// the quads of one depth band are contiguous along x (a row tile reads one
+// contiguous run per band) and a column tile reads CW consecutive quads per
+// band.  ldz (nzz rounded up to whole row tiles and 32) depths are stored;
+// the padding depths are never read as data.
 // Half spectra in kz (viscoacoustic path): float2 [nxx][ldk], kz in [0, nzh).
+__host__ __device__ inline size_t cq_q4(int nxx, int ix, int zq) { return ((size_t)zq * nxx + ix) * 4; }
 //
 // The fields of a job record.  G qualifies the device pointers: empty in the
 // host's JobD, the global address space in the kernels' JobG view of the same

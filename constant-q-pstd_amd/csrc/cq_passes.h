@@ -35,12 +35,12 @@ __device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__re
 // row helpers (real rows, packed)
 // ===========================================================================
 template <int Q>
-__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int ldz, int z0)
+__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int z0)
 {
     const int LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const float4 d = ld4(src + (size_t)ix * ldz + z0 + 4 * q);
+        const int q = e / nxx, ix = e - q * nxx;
+        const float4 d = ld4(src + cq_q4(nxx, ix, (z0 >> 2) + q));
         buf[(2 * q) * LS + cq_pad(ix)] = make_float2(d.x, d.y);
         buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(d.z, d.w);
     }
@@ -51,16 +51,16 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GL
 template <int Q>
 __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 {
-    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const int q = e / nxx, ix = e - q * nxx;
+        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
         const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         float4 dp = make_float4(a.x, a.y, b.x, b.y);
         const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float CQ_GLOBAL *php = J.phix + (size_t)sx * ldz + z0 + 4 * q;
+            float CQ_GLOBAL *php = J.phix + cq_q4(2 * pml, sx, (z0 >> 2) + q);
             const float hb = J.hbx[ix], ha = J.hax[ix];
             float4 ph = ld4(php);
             ph.x = hb * ph.x + ha * dp.x;
@@ -87,15 +87,15 @@ __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 template <int Q, bool TO_DIV = false>
 __device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, int z0)
 {
-    const int nxx = J.nxx, ldz = J.ldz, pml = J.pml, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const int q = e / nxx, ix = e - q * nxx;
+        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
         const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         float4 dv = make_float4(a.x, a.y, b.x, b.y);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float CQ_GLOBAL *psp = J.psix + (size_t)sx * ldz + z0 + 4 * q;
+            float CQ_GLOBAL *psp = J.psix + cq_q4(2 * pml, sx, (z0 >> 2) + q);
             const float ib = J.ibx[ix], ia = J.iax[ix];
             float4 ps = ld4(psp);
             ps.x = ib * ps.x + ia * dv.x;
@@ -128,9 +128,9 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ix < J.nxx) v = ld4(src + (size_t)ix * J.ldz + 4 * zq);
+        if (ix < J.nxx) v = ld4(src + cq_q4(J.nxx, ix, zq));
         float *line = col_line(bufs, nzz, c);
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -155,9 +155,9 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
-        const size_t g = (size_t)ix * J.ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         float *line = col_line(bufs, nzz, c);
         const float4 vz = ld4(J.vz + g), bz = ld4(J.dtbz + g);
         float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -168,7 +168,7 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
             float dp = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float CQ_GLOBAL *php = J.phiz + (size_t)ix * (2 * pml) + sz;
+                float CQ_GLOBAL *php = J.phiz + (size_t)sz * J.nxx + ix;
                 const float ph = J.hbz[iz] * (*php) + J.haz[iz] * dp;
                 *php = ph;
                 dp += ph;
@@ -189,7 +189,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         float *line = col_line(bufs, nzz, c);
         if (ix >= J.nxx) {
 #pragma unroll
@@ -197,7 +197,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
                 if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
             continue;
         }
-        const size_t g = (size_t)ix * J.ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         const float4 dvx = TO_DVZ ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(J.dvx + g);
         float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -207,7 +207,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
             float d = line[2 * cq_pad(iz)];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float CQ_GLOBAL *psp = J.psiz + (size_t)ix * (2 * pml) + sz;
+                float CQ_GLOBAL *psp = J.psiz + (size_t)sz * J.nxx + ix;
                 const float ps = J.ibz[iz] * (*psp) + J.iaz[iz] * d;
                 *psp = ps;
                 d += ps;
@@ -431,7 +431,7 @@ __device__ __forceinline__ void interior_accum(const JobG &J, float CQ_GLOBAL *d
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int zi = iz0 + k - J.pml;
-        if (zi >= 0 && zi < J.nz) dst[(size_t)xi * J.nz + zi] += f4get(a, k) * f4get(b, k);
+        if (zi >= 0 && zi < J.nz) dst[cq_q4(J.nxx, ix, iz0 >> 2) + k] += f4get(a, k) * f4get(b, k);
     }
 }
 
@@ -442,11 +442,11 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, z0 = it.y, LS = cq_line_stride(nxx);
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
-    row_fill<Q>(buf, nxx, J.div, ldz, z0);
+    row_fill<Q>(buf, nxx, J.div, z0);
     __syncthreads();
     cq_fft_fwd(J.px, buf, LS, L, tw, J.mx2);
     cq_fft_inv(J.px, buf, LS, L, tw);
@@ -457,8 +457,8 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     const bool rcv = J.role == 1;
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int ix = e / Q, q = e % Q;
-        const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+        const int q = e / nxx, ix = e - q * nxx;
+        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
         const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
         const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g), lz = ld4(J.lz + g),
                      d = ld4(J.div + g);
@@ -490,8 +490,8 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
         }
         __syncthreads();
         for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-            const int ix = e / Q, q = e % Q;
-            const size_t g = (size_t)ix * ldz + z0 + 4 * q;
+            const int q = e / nxx, ix = e - q * nxx;
+            const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
             const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
             const float4 pn = make_float4(a.x, a.y, b.x, b.y);
             st4(J.p + g, pn);
@@ -535,14 +535,14 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
     cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mz2);
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (4 * zq + k < nzz) f4set(o, k, line[2 * cq_pad(4 * zq + k)]);
-        st4(J.lz + (size_t)ix * J.ldz + 4 * zq, o);
+        st4(J.lz + cq_q4(J.nxx, ix, zq), o);
     }
 }
 
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
-    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
@@ -638,10 +638,10 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
-        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         const float4 gm = ld4(J.qg + g);
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -656,10 +656,10 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
-        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         const float4 y2 = ld4(J.qy2 + g), cc = ld4(J.qcc + g);
         float4 a = ld4(J.tot + g);
 #pragma unroll
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         float *line = col_line(bufs, nzz, c);
         if (ix >= J.nxx) {
 #pragma unroll
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
                 if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
             continue;
         }
-        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g), a = ld4(J.tot + g);
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
-    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2, ldz = J.ldz;
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nzz;
     float *bufs = reinterpret_cast<float *>(buf);
@@ -735,7 +735,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     // T waits in the scratch field (each thread re-reads only what it wrote)
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
             const int iz = 4 * zq + k;
             if (iz < nzz) f4set(a, k, line[2 * cq_pad(iz)]);
         }
-        st4(J.scr + (size_t)ix * ldz + 4 * zq, a);
+        st4(J.scr + cq_q4(J.nxx, ix, zq), a);
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     const float s = J.qsign;
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         float *line = col_line(bufs, nzz, c);
         if (ix >= J.nxx) {
 #pragma unroll
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
                 if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
             continue;
         }
-        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
                      qe = ld4(J.qe + g), tt = ld4(J.scr + g);
 #pragma unroll
@@ -789,10 +789,10 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         __syncthreads();
     }
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int c = e / nq, zq = e - c * nq, ix = x0 + c;
+        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
         const float *line = col_line(bufs, nzz, c);
-        const size_t g = (size_t)ix * ldz + 4 * zq;
+        const size_t g = cq_q4(J.nxx, ix, zq);
         float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -820,11 +820,11 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, c
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, ldz = J.ldz, z0 = it.y;
+    const int nxx = J.nxx, z0 = it.y;
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
-    row_fill<Q>(buf, nxx, J.p, ldz, z0);
+    row_fill<Q>(buf, nxx, J.p, z0);
     __syncthreads();
     cq_fft_fwd(J.px, buf, cq_line_stride(nxx), L, tw, J.mxp);
     cq_fft_inv(J.px, buf, cq_line_stride(nxx), L, tw);
