@@ -324,6 +324,7 @@ struct cq_batch_s {
     int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
+    int tk2 = 8;                              // kz-row tile of k_q_r2 (tk or 2 tk)
     int maxnxx = 0, maxnzz = 0;
     int t = 0, tmax = 0;                   // next global step; end of the last wavefield
     bool timing = false;
@@ -426,12 +427,12 @@ static bool r1_fused_for(int nxx, int tk);
 static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_only = nullptr)
 {
     cq_ctx ctx = b->ctx;
-    const int cw = is_q(j.kind) ? b->cwq : b->cw, tk = b->tk, tz = b->tz;
+    const int cw = is_q(j.kind) ? b->cwq : b->cw, tk = b->tk, tk2 = b->tk2, tz = b->tz;
     const bool q = is_q(j.kind), rtm = is_rtm(j.kind);
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = ((nzz + tz - 1) / tz * tz + 31) & ~31;   // whole row tiles, 128-B rows
     const int nzh = nzz / 2 + 1;
-    const int ldk = (nzh + tk - 1) / tk * tk;
+    const int ldk = (nzh + tk2 - 1) / tk2 * tk2;   // whole tiles of both kz-row passes (tk2 = tk or 2 tk)
     const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
     const size_t N = (size_t)nxx * ldz;
     const int ntile_x = (nxx + cw - 1) / cw, ntile_z = (nzz + tz - 1) / tz;
@@ -453,9 +454,9 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.gather = rtm ? 0 : 1;
     base.r1_fused = q && r1_fused_for(nxx, tk) ? 1 : 0;
     int rc;
-    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 3 tk
+    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 2 tk
     // (fused k_q_r1) or tk lines of nxx; column tiles cw/2 lines of nzz
-    const int need_x = std::max((tz / 2) * nxx, q ? (base.r1_fused ? 3 : 1) * tk * nxx : 0),
+    const int need_x = std::max((tz / 2) * nxx, q ? std::max((base.r1_fused ? 2 : 1) * tk, tk2) * nxx : 0),
               need_z = (cw / 2) * nzz;
     if ((rc = get_plan(ctx, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
     if ((rc = get_plan(ctx, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
@@ -714,16 +715,17 @@ static const int kTzCand[] = {16, 12, 8, 4};
 static const int kCwCand[] = {32, 24, 16, 12, 8, 4, 2};
 static const int kTkCand[] = {8, 4, 2};
 
-// k_q_r1 inverts its three symbols together (3 tk lines) when a plan and the
-// LDS budget allow it, else one at a time (2 tk lines: spectrum + work).
+// k_q_r1 holds 3 tk lines and transforms 2 tk at a time (the div and p
+// spectra, then two symbols at a time) when the LDS budget allows it, else
+// inverts one symbol at a time (2 tk lines: spectrum + work, tk at a time).
 static bool r1_fused_for(int nxx, int tk)
 {
-    return has_plan(nxx, 3 * tk * nxx) && lds_lines(nxx, 3 * tk) <= kLdsTile;
+    return has_plan(nxx, 2 * tk * nxx) && lds_lines(nxx, 3 * tk) <= kLdsTile;
 }
 static size_t r1_lds(int nxx, int tk) { return lds_lines(nxx, r1_fused_for(nxx, tk) ? 3 * tk : 2 * tk); }
 static bool r1_ok(int nxx, int tk, size_t lim)
 {
-    return (has_plan(nxx, 3 * tk * nxx) && lds_lines(nxx, 3 * tk) <= lim) ||
+    return (has_plan(nxx, 2 * tk * nxx) && lds_lines(nxx, 3 * tk) <= lim) ||
            (has_plan(nxx, tk * nxx) && lds_lines(nxx, 2 * tk) <= lim);
 }
 
@@ -737,7 +739,7 @@ static void set_lds(cq_batch b, int ax, int az, int qx, int qz, const std::vecto
     b->lds[P_QR1] = 0;
     for (int n : qxs) b->lds[P_QR1] = std::max(b->lds[P_QR1], r1_lds(n, b->tk));
     b->lds[P_QC2] = lds_lines(qz, b->cwq / 2);
-    b->lds[P_QR2] = lds_lines(qx, b->tk);
+    b->lds[P_QR2] = lds_lines(qx, b->tk2);
     b->lds[P_QC3] = lds_lines(qz, b->cwq / 2);
     b->lds[P_QR3] = lds_lines(qx, b->tz / 2);
 }
@@ -786,6 +788,10 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
     b->cw = kCwCand[ic];
     b->cwq = kCwCand[icq];
     b->tk = kTkCand[ik];
+    // k_q_r2 takes two kz-row tiles at once where every length allows it
+    b->tk2 = 2 * b->tk;
+    for (int n : qxs)
+        if (!has_plan(n, b->tk2 * n) || lds_lines(n, b->tk2) > kLdsTile) b->tk2 = b->tk;
     b->maxnxx = std::max(ax, qx);
     b->maxnzz = std::max(az, qz);
     set_lds(b, ax, az, qx, qz, qxs);
@@ -829,10 +835,8 @@ static int rebuild_tables(cq_batch b)
                     it[P_QC2].push_back(make_int2(e, x0));
                     it[P_QC3].push_back(make_int2(e, x0));
                 }
-                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) {
-                    it[P_QR1].push_back(make_int2(e, k0));
-                    it[P_QR2].push_back(make_int2(e, k0));
-                }
+                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) it[P_QR1].push_back(make_int2(e, k0));
+                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk2) it[P_QR2].push_back(make_int2(e, k0));
                 for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_QR3].push_back(make_int2(e, z0));
             }
         }
@@ -910,9 +914,10 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
      : W == 8  ? launch(b, K<8>, CQ_ARGS)                                          \
      : W == 4  ? launch(b, K<4>, CQ_ARGS)                                          \
                : launch(b, K<2>, CQ_ARGS))
-#define CQ_DISPATCH_TK(K)                                                          \
-    (b->tk == 8 ? launch(b, K<8>, CQ_ARGS)                                         \
-     : b->tk == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
+#define CQ_DISPATCH_TK(K, T)                                                       \
+    (T == 16  ? launch(b, K<16>, CQ_ARGS)                                          \
+     : T == 8 ? launch(b, K<8>, CQ_ARGS)                                           \
+     : T == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
 
 static int launch_pass(cq_batch b, int pass, int role, int step)
 {
@@ -920,9 +925,9 @@ static int launch_pass(cq_batch b, int pass, int role, int step)
     case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
     case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, b->cw);
     case P_QC1: return CQ_DISPATCH_CW(k_q_c1, b->cwq);
-    case P_QR1: return CQ_DISPATCH_TK(k_q_r1);
+    case P_QR1: return CQ_DISPATCH_TK(k_q_r1, b->tk);
     case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
-    case P_QR2: return CQ_DISPATCH_TK(k_q_r2);
+    case P_QR2: return CQ_DISPATCH_TK(k_q_r2, b->tk2);
     case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
     case P_QR3: return CQ_DISPATCH_TZ(k_q_r3);
     }
@@ -1130,7 +1135,7 @@ static int server_tiles(cq_batch b, int mx, int mz)
     int iz = 0, ic = 0, ik = 0;
     auto lim = [&](size_t lds, int pts, size_t budget) { return lds > budget || pts > cap; };
     auto r1_bad = [&](int k, size_t budget) {
-        return lim(lds_lines(mx, 3 * k), 3 * k * mx, budget) && lim(lds_lines(mx, 2 * k), k * mx, budget);
+        return lim(lds_lines(mx, 3 * k), 2 * k * mx, budget) && lim(lds_lines(mx, 2 * k), k * mx, budget);
     };
     while (iz < 3 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
     while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz, kLdsTile)) ic++;
@@ -1142,6 +1147,7 @@ static int server_tiles(cq_batch b, int mx, int mz)
     if (lim(lds_lines(mx, b->tz / 2), (b->tz / 2) * mx, kLdsMax) || lim(lds_lines(mz, b->cw / 2), (b->cw / 2) * mz, kLdsMax) ||
         r1_bad(b->tk, kLdsMax))
         return set_err(CQ_EUNSUPPORTED, "server windows up to %dx%d are too large for one workgroup", mx, mz);
+    b->tk2 = lim(lds_lines(mx, 2 * b->tk), 2 * b->tk * mx, kLdsTile) ? b->tk : 2 * b->tk;
     b->maxnxx = mx;
     b->maxnzz = mz;
     set_lds(b, mx, mz, mx, mz, {});

@@ -315,25 +315,60 @@ __device__ __forceinline__ float frac_symbol(int which, float k2)
     return which == 0 ? r : k2 * r;
 }
 
-// The three dispersion symbols at once: spectrum line l (of TK) becomes
-// line l x |k|^(1/16) / N, line TK + l x |k|^2 / N and line 2 TK + l x
-// |k|^(33/16) / N (the frac_symbol values 0, 1, 2).
+// |k|^2 on the float wavenumber grid at kx (FFT order) and kz = kzi.
+__device__ __forceinline__ float krow_k2(const JobG &J, int kx, int kzi)
+{
+    const int ikx = kx <= J.nxx / 2 ? kx : J.nxx - kx;
+    const float fkx = (float)ikx * J.dkx;
+    const float fkz = (float)kzi * J.dkz;
+    return __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
+}
+
+// The multiplier of the previous step's p: symbol 3 (|k|^1.01), or, which = 4,
+// the stabilised compensation multiplier of Q-RTM
+//   S = k - 2 e k (1 + tc dt k) / (exp(-tc tt k) + e exp(tc dt k)),  k = |k|^1.01,
+// e = eta * 1e-6, tc = max tao, tt = t_aux dt (multt_kernel,
+// sgpstd-qrtm/acsglw.cu:1895-1924).
+__device__ __forceinline__ float aux_symbol_of(const JobG &J, int which, float k2, float tt)
+{
+    if (which != 4) return frac_symbol(which, k2);
+    const float k = frac_symbol(3, k2);
+    const float es = J.eta * 1.0e-6f;
+    const float a = J.taoc * J.dt * k;
+    return k - 2.0f * es * k * (1.0f + a) / (expf(-J.taoc * tt * k) + es * expf(a));
+}
+
+// Fused R1, first round.  Lines 0.. hold the div spectrum, lines TK.. the p
+// spectrum (TK lines each): p becomes aux = p x aux symbol / N in place and
+// lines 2 TK.. receive K = div x |k|^2 / N.
 template <int TK>
-__device__ __forceinline__ void krow_mult3(const JobG &J, float2 *buf, int k0)
+__device__ __forceinline__ void krow_round1(const JobG &J, float2 *buf, int k0, int which, float tt)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
-        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
-        const float fkx = (float)ikx * J.dkx;
-        const float fkz = (float)(k0 + l) * J.dkz;
-        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
-        const float r = exp2f(log2f(k2) * (1.0f / 32.0f));
-        const float ma = r * J.invN, mk = k2 * J.invN, mb = (k2 * r) * J.invN;
+        const float k2 = krow_k2(J, kx, k0 + l);
+        const float ms = aux_symbol_of(J, which, k2, tt) * J.invN, mk = k2 * J.invN;
+        const float2 v = buf[pe], w = buf[TK * LS + pe];
+        buf[TK * LS + pe] = make_float2(w.x * ms, w.y * ms);
+        buf[2 * TK * LS + pe] = make_float2(v.x * mk, v.y * mk);
+    }
+}
+
+// Fused R1, second round: the div spectrum (lines 0..) becomes A = div x
+// |k|^(1/16) / N in place and lines TK.. receive B = div x |k|^(33/16) / N.
+template <int TK>
+__device__ __forceinline__ void krow_round2(const JobG &J, float2 *buf, int k0)
+{
+    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const float k2 = krow_k2(J, kx, k0 + l);
+        const float r = frac_symbol(0, k2);
+        const float ma = r * J.invN, mb = (k2 * r) * J.invN;
         const float2 v = buf[pe];
         buf[pe] = make_float2(v.x * ma, v.y * ma);
-        buf[TK * LS + pe] = make_float2(v.x * mk, v.y * mk);
-        buf[2 * TK * LS + pe] = make_float2(v.x * mb, v.y * mb);
+        buf[TK * LS + pe] = make_float2(v.x * mb, v.y * mb);
     }
 }
 
@@ -344,41 +379,20 @@ __device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
-        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
-        const float fkx = (float)ikx * J.dkx;
-        const float fkz = (float)(k0 + l) * J.dkz;
-        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
-        const float m = frac_symbol(which, k2) * J.invN;
+        const float m = frac_symbol(which, krow_k2(J, kx, k0 + l)) * J.invN;
         const float2 v = buf[pe];
         buf[dst * LS + pe] = make_float2(v.x * m, v.y * m);
     }
 }
 
-// buf[l][kx] *= symbol(kx, k0 + l) / N.
-// which = 4: the stabilised compensation multiplier of Q-RTM,
-//   S = k - 2 e k (1 + tc dt k) / (exp(-tc tt k) + e exp(tc dt k)),  k = |k|^1.01,
-// e = eta * 1e-6, tc = max tao, tt = t_aux dt (multt_kernel,
-// sgpstd-qrtm/acsglw.cu:1895-1924).
+// buf[l][kx] *= aux_symbol_of(which) (kx, k0 + l) / N.
 template <int TK>
 __device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, int which, float tt = 0.f)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
-        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
-        const float fkx = (float)ikx * J.dkx;
-        const float fkz = (float)(k0 + l) * J.dkz;
-        const float k2 = __fadd_rn(__fmul_rn(fkx, fkx), __fmul_rn(fkz, fkz));
-        float m;
-        if (which == 4) {
-            const float k = frac_symbol(3, k2);
-            const float es = J.eta * 1.0e-6f;
-            const float a = J.taoc * J.dt * k;
-            m = k - 2.0f * es * k * (1.0f + a) / (expf(-J.taoc * tt * k) + es * expf(a));
-        } else {
-            m = frac_symbol(which, k2);
-        }
-        m *= J.invN;
+        const float m = aux_symbol_of(J, which, krow_k2(J, kx, k0 + l), tt) * J.invN;
         const float2 v = buf[pe];
         buf[pe] = make_float2(v.x * m, v.y * m);
     }
@@ -583,21 +597,36 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, k0 = it.y, LS = cq_line_stride(nxx);
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nxx;   // 3 TK lines: A, K, B
+    float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
+    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566);
+    // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
+    const int aux_symbol = J.stab ? 4 : 3;
+    const float tt = (float)t_aux * J.dt;
+    if (J.r1_fused) {
+        // 3 TK lines: the div and p spectra transformed together, then two
+        // rounds of two symbols each, inverse-transformed together
+        krow_fill<TK>(buf, J, J.s1, k0);
+        krow_fill<TK>(buf + TK * LS, J, J.s2, k0);
+        __syncthreads();
+        cq_fft_fwd(J.px, buf, LS, 2 * TK, tw, nullptr);
+        krow_round1<TK>(J, buf, k0, aux_symbol, tt);   // aux (lines TK..), K (lines 2 TK..)
+        __syncthreads();
+        cq_fft_inv(J.px, buf + TK * LS, LS, 2 * TK, tw);
+        krow_store<TK>(buf + TK * LS, J, J.s2, k0);
+        krow_store<TK>(buf + 2 * TK * LS, J, J.s3, k0);
+        __syncthreads();
+        krow_round2<TK>(J, buf, k0);                   // A (lines 0..), B (lines TK..)
+        __syncthreads();
+        cq_fft_inv(J.px, buf, LS, 2 * TK, tw);
+        krow_store<TK>(buf, J, J.s1, k0);
+        krow_store<TK>(buf + TK * LS, J, J.s4, k0);
+        return;
+    }
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
     cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
-    // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566)
-    if (J.r1_fused) {
-        // inverse-transformed together (3 TK lines)
-        krow_mult3<TK>(J, buf, k0);
-        __syncthreads();
-        cq_fft_inv(J.px, buf, LS, 3 * TK, tw);
-        krow_store<TK>(buf, J, J.s1, k0);
-        krow_store<TK>(buf + TK * LS, J, J.s3, k0);
-        krow_store<TK>(buf + 2 * TK * LS, J, J.s4, k0);
-    } else {
+    {
         // one symbol at a time (long lines: 2 TK lines of LDS)
         float2 CQ_GLOBAL *outs[3] = {J.s1, J.s3, J.s4};
         for (int w = 0; w < 3; w++) {
@@ -609,13 +638,12 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
         }
     }
     __syncthreads();
-    // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
     krow_fill<TK>(buf, J, J.s2, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
-    krow_mult<TK>(J, buf, k0, J.stab ? 4 : 3, (float)t_aux * J.dt);
+    cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
+    krow_mult<TK>(J, buf, k0, aux_symbol, tt);
     __syncthreads();
-    cq_fft_inv(J.px, buf, cq_line_stride(nxx), TK, tw);
+    cq_fft_inv(J.px, buf, LS, TK, tw);
     krow_store<TK>(buf, J, J.s2, k0);
 }
 
