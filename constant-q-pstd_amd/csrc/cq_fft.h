@@ -32,13 +32,19 @@ struct CqPlan {
     int rad[CQ_MAXST];
 };
 
-// LDS line layout: value i of a line sits at slot cq_pad(i), one unused slot
-// after every 16 values, so that the stride-R stores of the early Stockham
-// stages (and stride-4 accesses of the column helpers) spread over the LDS
-// banks instead of landing on one.  Lines are cq_line_stride(n) slots apart
-// (odd, so consecutive lines start on different banks).
-__host__ __device__ constexpr int cq_pad(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int cq_line_stride(int n) { return cq_pad(n) | 1; }
+// LDS line layout: a line holds its n values in order (slot i = value i) and
+// lines are cq_line_stride(n) slots apart (odd, so that consecutive lines
+// start on different banks).  One exception: between the first and second
+// stage of a transform whose first radix equals the length's pad period P
+// (16 when 16 | n, else 8 when 8 | n), value i sits at slot i + i / P.  The
+// first stage's outputs, R apart in value order, then land R + 1 apart (no
+// bank conflicts), and both stages still address their R values at one
+// constant stride.
+__host__ __device__ constexpr int cq_pad_period(int n) { return n % 16 == 0 ? 16 : n % 8 == 0 ? 8 : 0; }
+__host__ __device__ constexpr int cq_line_stride(int n)
+{
+    return (n + (cq_pad_period(n) ? n / cq_pad_period(n) : 0)) | 1;
+}
 
 // ---------------------------------------------------------------------------
 // compile-time trigonometry for the butterfly constants
@@ -240,7 +246,10 @@ extern __shared__ float2 cq_lds[];
 //        slots (no bank conflicts)
 // Ns   : product of the radices of the previous stages
 // mh   : optional multiplier (global, length n) applied to the outputs
-template <int R, bool INV>
+// MODE : 0 values in order on both sides; 1 first stage of several, R = pad
+//        period, outputs in the padded layout; 2 second stage after a MODE 1
+//        stage (Ns = pad period), inputs in the padded layout
+template <int R, bool INV, int MODE>
 __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
                                       const float2 CQ_GLOBAL *__restrict__ mh)
 {
@@ -249,9 +258,17 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     constexpr int NB = cq_nb(R);
     const int m = n / R;
     const int total = L * m;
-    // exact quotients by float reciprocal: operands are below 2^14, so
-    // (i + 0.5) / d never lies within rounding distance of an integer
-    const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
+    // exact quotients by the hardware reciprocal (1 ulp): operands are below
+    // 2^14, so (i + 0.5) / d lies at least 0.5 / d from an integer while the
+    // product is off by less than 2^-8 / d
+    const float inv_m = __builtin_amdgcn_rcpf((float)m), inv_ns = __builtin_amdgcn_rcpf((float)Ns);
+    // slots between a butterfly's inputs: m, or m + m / Ns in the padded
+    // layout (the period Ns divides m)
+    const int rs = MODE == 2 ? m + m / Ns : m;
+    const int psh = MODE == 2 ? 31 - __builtin_clz(Ns) : 0;
+    // the first stage (Ns = 1) has only unit twiddles; later stages multiply
+    // every input, the k = 0 entries of the table being exactly (1, 0)
+    const bool twiddle = MODE != 1 && Ns > 1;
     float2 v[NB][R];
     int base[NB], kk[NB];
 #pragma unroll
@@ -263,19 +280,22 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
             const int l = (int)(((float)idx + 0.5f) * inv_m);
             const int j = idx - l * m;
             const int k = j - (int)(((float)j + 0.5f) * inv_ns) * Ns;
-            const float2 *src = buf + l * LS;
+            const float2 *src = buf + l * LS + (MODE == 2 ? j + (j >> psh) : j);
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                float2 x = src[cq_pad(j + r * m)];
-                if (r > 0 && k != 0) {
-                    const float2 w = tw[(r - 1) * Ns + k];
-                    x = INV ? cmulc(x, w) : cmul(x, w);
+            for (int r = 0; r < R; r++) v[b][r] = src[r * rs];
+            if (twiddle) {
+                const float2 *twk = tw + k;
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const float2 w = twk[(r - 1) * Ns];
+                    v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
                 }
-                v[b][r] = x;
             }
             Dft<R, INV>::run(v[b]);
-            base[b] = l * LS;
+            // output q is value kk + q Ns: slot kk + q Ns, or, padded (Ns = 1,
+            // k = 0), j (R + 1) + q
             kk[b] = (j - k) * R + k;
+            base[b] = l * LS + (MODE == 1 ? j * (R + 1) : kk[b]);
             if (mh != nullptr) {
 #pragma unroll
                 for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk[b] + q * Ns]);
@@ -286,31 +306,46 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         if (base[b] >= 0) {
+            float2 *dst = buf + base[b];
 #pragma unroll
-            for (int q = 0; q < R; q++) buf[base[b] + cq_pad(kk[b] + q * Ns)] = v[b][q];
+            for (int q = 0; q < R; q++) dst[q * Ns] = v[b][q];
         }
     }
     __syncthreads();
 }
 
+#define CQ_STAGE_CASES(INV, MODE)                                                   \
+    case 2: cq_stage<2, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 3: cq_stage<3, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 4: cq_stage<4, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 5: cq_stage<5, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 7: cq_stage<7, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 8: cq_stage<8, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 9: cq_stage<9, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
+    case 11: cq_stage<11, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
+    case 13: cq_stage<13, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
+    case 16: cq_stage<16, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
+    case 17: cq_stage<17, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
+    default: break;
+
 template <bool INV>
-__device__ __forceinline__ void cq_stage_rt(int R, int bo, int LS, int n, int L, int Ns, int to,
+__device__ __forceinline__ void cq_stage_rt(int R, int mode, int bo, int LS, int n, int L, int Ns, int to,
                                             const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    switch (R) {
-    case 2: cq_stage<2, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 3: cq_stage<3, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 4: cq_stage<4, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 5: cq_stage<5, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 7: cq_stage<7, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 8: cq_stage<8, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 9: cq_stage<9, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 11: cq_stage<11, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 13: cq_stage<13, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 16: cq_stage<16, INV>(bo, LS, n, L, Ns, to, mh); break;
-    case 17: cq_stage<17, INV>(bo, LS, n, L, Ns, to, mh); break;
-    default: break;
+    if (mode == 1) {
+        if (R == 16) cq_stage<16, INV, 1>(bo, LS, n, L, Ns, to, mh);
+        else cq_stage<8, INV, 1>(bo, LS, n, L, Ns, to, mh);
+    } else if (mode == 2) {
+        switch (R) { CQ_STAGE_CASES(INV, 2) }
+    } else {
+        switch (R) { CQ_STAGE_CASES(INV, 0) }
     }
+}
+
+// Layout mode of a plan's first stage (the second follows: 1 -> 2, 0 -> 0).
+__device__ __forceinline__ int cq_first_mode(const CqPlan &p)
+{
+    return p.ns > 1 && p.rad[0] == cq_pad_period(p.n) ? 1 : 0;
 }
 
 // Twiddle table of a plan (n - 1 entries, built on the host by the same rule):
@@ -324,12 +359,13 @@ __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS,
 {
     const int bo = (int)(buf - cq_lds);
     int to = (int)(tw - cq_lds);
-    int Ns = 1;
+    int Ns = 1, mode = cq_first_mode(p);
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<false>(R, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
+        cq_stage_rt<false>(R, mode, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
         to += (R - 1) * Ns;
         Ns *= R;
+        mode = mode == 1 ? 2 : 0;
     }
 }
 
@@ -338,12 +374,13 @@ __device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS,
 {
     const int bo = (int)(buf - cq_lds);
     int to = (int)(tw - cq_lds);
-    int Ns = 1;
+    int Ns = 1, mode = cq_first_mode(p);
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<true>(R, bo, LS, p.n, L, Ns, to, nullptr);
+        cq_stage_rt<true>(R, mode, bo, LS, p.n, L, Ns, to, nullptr);
         to += (R - 1) * Ns;
         Ns *= R;
+        mode = mode == 1 ? 2 : 0;
     }
 }
 

@@ -41,8 +41,8 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GL
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int q = e / nxx, ix = e - q * nxx;
         const float4 d = ld4(src + cq_q4(nxx, ix, (z0 >> 2) + q));
-        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(d.x, d.y);
-        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(d.z, d.w);
+        buf[(2 * q) * LS + ix] = make_float2(d.x, d.y);
+        buf[(2 * q + 1) * LS + ix] = make_float2(d.z, d.w);
     }
 }
 
@@ -55,7 +55,7 @@ __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int q = e / nxx, ix = e - q * nxx;
         const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
+        const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
         float4 dp = make_float4(a.x, a.y, b.x, b.y);
         const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
         if (ix < pml || ix >= nxx - pml) {
@@ -76,8 +76,8 @@ __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
         vn.z = vx.z + bx.z * dp.z;
         vn.w = vx.w + bx.w * dp.w;
         st4(J.vx + g, vn);
-        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(vn.x, vn.y);
-        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(vn.z, vn.w);
+        buf[(2 * q) * LS + ix] = make_float2(vn.x, vn.y);
+        buf[(2 * q + 1) * LS + ix] = make_float2(vn.z, vn.w);
     }
 }
 
@@ -91,7 +91,7 @@ __device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, 
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int q = e / nxx, ix = e - q * nxx;
         const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
+        const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
         float4 dv = make_float4(a.x, a.y, b.x, b.y);
         if (ix < pml || ix >= nxx - pml) {
             const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
@@ -134,7 +134,7 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float
         float *line = col_line(bufs, nzz, c);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = f4get(v, k);
+            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
     }
 }
 
@@ -165,7 +165,7 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float dp = line[2 * cq_pad(iz)];
+            float dp = line[2 * iz];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
                 float CQ_GLOBAL *php = J.phiz + (size_t)sz * J.nxx + ix;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
             }
             const float v = f4get(vz, k) + f4get(bz, k) * dp;
             f4set(vn, k, v);
-            line[2 * cq_pad(iz)] = v;
+            line[2 * iz] = v;
         }
         st4(J.vz + g, vn);
     }
@@ -194,7 +194,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
         if (ix >= J.nxx) {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
             continue;
         }
         const size_t g = cq_q4(J.nxx, ix, zq);
@@ -204,7 +204,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            float d = line[2 * cq_pad(iz)];
+            float d = line[2 * iz];
             if (iz < pml || iz >= nzz - pml) {
                 const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
                 float CQ_GLOBAL *psp = J.psiz + (size_t)sz * J.nxx + ix;
@@ -214,7 +214,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
             }
             const float s = f4get(dvx, k) + d;
             f4set(dv, k, s);
-            line[2 * cq_pad(iz)] = s;
+            line[2 * iz] = s;
         }
         st4((TO_DVZ ? J.dvz : J.div) + g, dv);
     }
@@ -234,15 +234,15 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
         const int l = e / nzh, k = e - l * nzh;
         const int k2 = k ? nzz - k : 0;
         float2 *line = buf + l * cq_line_stride(nzz);
-        const float2 zk = line[cq_pad(k)], zm = line[cq_pad(k2)];
+        const float2 zk = line[k], zm = line[k2];
         const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
         const int ix0 = x0 + 2 * l;
         if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
         if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
         if (mh != nullptr) {
-            line[cq_pad(k)] = cmul(zk, mh[k]);
-            if (k2 != k) line[cq_pad(k2)] = cmul(zm, mh[k2]);
+            line[k] = cmul(zk, mh[k]);
+            if (k2 != k) line[k2] = cmul(zm, mh[k2]);
         }
     }
 }
@@ -267,8 +267,8 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
             h0.y = 0.f;
             h1.y = 0.f;
         }
-        line[cq_pad(k)] = make_float2(h0.x - h1.y, h0.y + h1.x);
-        if (k2 != k) line[cq_pad(k2)] = make_float2(h0.x + h1.y, h1.x - h0.y);
+        line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
+        if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
     }
 }
 
@@ -283,8 +283,8 @@ __device__ __forceinline__ void krow_fill(float2 *buf, const JobG &J, const floa
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
         const float4 v = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
-        buf[(2 * h) * LS + cq_pad(ix)] = make_float2(v.x, v.y);
-        buf[(2 * h + 1) * LS + cq_pad(ix)] = make_float2(v.z, v.w);
+        buf[(2 * h) * LS + ix] = make_float2(v.x, v.y);
+        buf[(2 * h + 1) * LS + ix] = make_float2(v.z, v.w);
     }
 }
 
@@ -295,7 +295,7 @@ __device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, flo
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
         const int ix = e / H, h = e % H;
-        const float2 a = buf[(2 * h) * LS + cq_pad(ix)], b = buf[(2 * h + 1) * LS + cq_pad(ix)];
+        const float2 a = buf[(2 * h) * LS + ix], b = buf[(2 * h + 1) * LS + ix];
         *reinterpret_cast<float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
     }
 }
@@ -346,7 +346,7 @@ __device__ __forceinline__ void krow_round1(const JobG &J, float2 *buf, int k0, 
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
         const float k2 = krow_k2(J, kx, k0 + l);
         const float ms = aux_symbol_of(J, which, k2, tt) * J.invN, mk = k2 * J.invN;
         const float2 v = buf[pe], w = buf[TK * LS + pe];
@@ -362,7 +362,7 @@ __device__ __forceinline__ void krow_round2(const JobG &J, float2 *buf, int k0)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
         const float k2 = krow_k2(J, kx, k0 + l);
         const float r = frac_symbol(0, k2);
         const float ma = r * J.invN, mb = (k2 * r) * J.invN;
@@ -378,7 +378,7 @@ __device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
         const float m = frac_symbol(which, krow_k2(J, kx, k0 + l)) * J.invN;
         const float2 v = buf[pe];
         buf[dst * LS + pe] = make_float2(v.x * m, v.y * m);
@@ -391,7 +391,7 @@ __device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, in
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + cq_pad(kx);
+        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
         const float m = aux_symbol_of(J, which, krow_k2(J, kx, k0 + l), tt) * J.invN;
         const float2 v = buf[pe];
         buf[pe] = make_float2(v.x * m, v.y * m);
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
         const int q = e / nxx, ix = e - q * nxx;
         const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-        const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
+        const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
         const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g), lz = ld4(J.lz + g),
                      d = ld4(J.div + g);
         float4 pn;
@@ -489,8 +489,8 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
             st4(J.p + g, pn);
             if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, z0 + 4 * q, pn, pn);
         }
-        buf[(2 * q) * LS + cq_pad(ix)] = make_float2(pn.x, pn.y);
-        buf[(2 * q + 1) * LS + cq_pad(ix)] = make_float2(pn.z, pn.w);
+        buf[(2 * q) * LS + ix] = make_float2(pn.x, pn.y);
+        buf[(2 * q + 1) * LS + ix] = make_float2(pn.z, pn.w);
     }
     __syncthreads();
     if (rcv) {
@@ -499,14 +499,14 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
         const int tile = z0 / TZ;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT) {
             const int dz = J.inj_z[r] - z0;
-            float *f = reinterpret_cast<float *>(buf + (dz >> 1) * LS + cq_pad(J.inj_x[r])) + (dz & 1);
+            float *f = reinterpret_cast<float *>(buf + (dz >> 1) * LS + J.inj_x[r]) + (dz & 1);
             *f = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         }
         __syncthreads();
         for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
             const int q = e / nxx, ix = e - q * nxx;
             const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-            const float2 a = buf[(2 * q) * LS + cq_pad(ix)], b = buf[(2 * q + 1) * LS + cq_pad(ix)];
+            const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
             const float4 pn = make_float4(a.x, a.y, b.x, b.y);
             st4(J.p + g, pn);
             interior_accum(J, J.img, ix, z0 + 4 * q, pn, ld4(J.psrc + g));
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
     col_fill<CW>(bufs, J, J.p, x0);
     __syncthreads();
     // receiver gather of the updated p (acps4.cu:611-621)
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * cq_pad(z)]; });
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
     cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzp);
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
     col_vz_phase<CW>(J, bufs, x0);
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) f4set(o, k, line[2 * cq_pad(4 * zq + k)]);
+            if (4 * zq + k < nzz) f4set(o, k, line[2 * (4 * zq + k)]);
         st4(J.lz + cq_q4(J.nxx, ix, zq), o);
     }
 }
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(gm, k)) * line[2 * cq_pad(iz)]);
+            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(gm, k)) * line[2 * iz]);
         }
         st4(J.tot + g, a);
     }
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * cq_pad(iz)]));
+            if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * iz]));
         }
         st4(J.tot + g, a);
     }
@@ -707,7 +707,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
         if (ix >= J.nxx) {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
             continue;
         }
         const size_t g = cq_q4(J.nxx, ix, zq);
@@ -717,10 +717,10 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
-            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * cq_pad(iz)] -
+            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * iz] -
                              f4get(a, k);
             f4set(o, k, tv);
-            line[2 * cq_pad(iz)] = tv;
+            line[2 * iz] = tv;
         }
         st4(J.tot + g, o);
     }
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, line[2 * cq_pad(iz)]);
+            if (iz < nzz) f4set(a, k, line[2 * iz]);
         }
         st4(J.scr + cq_q4(J.nxx, ix, zq), a);
     }
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         if (ix >= J.nxx) {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * cq_pad(4 * zq + k)] = 0.f;
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
             continue;
         }
         const size_t g = cq_q4(J.nxx, ix, zq);
@@ -802,9 +802,9 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
             const int iz = 4 * zq + k;
             if (iz >= nzz) continue;
             float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * f4get(tt, k)) -
-                      s * f4get(qe, k) * line[2 * cq_pad(iz)];
+                      s * f4get(qe, k) * line[2 * iz];
             if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
-            line[2 * cq_pad(iz)] = v;
+            line[2 * iz] = v;
         }
     }
     __syncthreads();
@@ -813,7 +813,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         // sgpstd-qrtm/acsglw.cu:1809-1841)
         const int tile = x0 / CW;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT)
-            col_line(bufs, nzz, J.inj_x[r] - x0)[2 * cq_pad(J.inj_z[r])] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+            col_line(bufs, nzz, J.inj_x[r] - x0)[2 * J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         __syncthreads();
     }
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
@@ -824,12 +824,12 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
         float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) f4set(pn, k, line[2 * cq_pad(4 * zq + k)]);
+            if (4 * zq + k < nzz) f4set(pn, k, line[2 * (4 * zq + k)]);
         st4(J.p + g, pn);
         if (rcv) interior_accum(J, J.img, ix, 4 * zq, pn, ld4(J.psrc + g));
         else if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, 4 * zq, pn, pn);
     }
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * cq_pad(z)]; });
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
     __syncthreads();
     cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
