@@ -253,6 +253,14 @@ template <int R, bool INV, int MODE>
 __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
                                       const float2 CQ_GLOBAL *__restrict__ mh)
 {
+    // the arguments are uniform but arrive in vector registers: move them to
+    // scalar registers so that the index and stride arithmetic runs there
+    bo = __builtin_amdgcn_readfirstlane(bo);
+    LS = __builtin_amdgcn_readfirstlane(LS);
+    n = __builtin_amdgcn_readfirstlane(n);
+    L = __builtin_amdgcn_readfirstlane(L);
+    Ns = __builtin_amdgcn_readfirstlane(Ns);
+    to = __builtin_amdgcn_readfirstlane(to);
     float2 *const buf = cq_lds + bo;
     const float2 *const tw = cq_lds + to;
     constexpr int NB = cq_nb(R);

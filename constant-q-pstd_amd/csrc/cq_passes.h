@@ -26,6 +26,40 @@ __device__ __forceinline__ void f4set(float4 &v, int k, float s)
     else v.w = s;
 }
 
+// The loop e = threadIdx.x, + CQ_NT, ... < total in groups of U iterations:
+// pre(e) of all U first (the global loads; it returns what it loaded), then
+// post(e, loaded) of all U, so that U iterations' loads are in flight
+// together rather than one iteration's at a time.
+template <int U, typename Pre, typename Post>
+__device__ __forceinline__ void cq_loop(int total, Pre pre, Post post)
+{
+    using T = decltype(pre(0));
+    for (int e0 = threadIdx.x; e0 < total; e0 += U * CQ_NT) {
+        T d[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (e0 + u * CQ_NT < total) d[u] = pre(e0 + u * CQ_NT);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (e0 + u * CQ_NT < total) post(e0 + u * CQ_NT, d[u]);
+    }
+}
+
+// e / n for 0 <= e < 2^21 by the hardware reciprocal of n (inv = 1 / n as
+// __builtin_amdgcn_rcpf gives it): (e + 0.5) / n lies at least 0.5 / n from
+// an integer, the product is off by less than (e + 0.5) / n * 2^-22.
+__device__ __forceinline__ int cq_div(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
+
+struct F4x2 {
+    float4 a, b;
+};
+struct F4x3 {
+    float4 a, b, c;
+};
+struct F4x5 {
+    float4 a, b, c, d, e;
+};
+
 __device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__restrict__ g, int n)
 {
     for (int i = threadIdx.x; i < n; i += CQ_NT) tw[i] = g[i];
@@ -38,12 +72,14 @@ template <int Q>
 __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int z0)
 {
     const int LS = cq_line_stride(nxx);
-    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int q = e / nxx, ix = e - q * nxx;
-        const float4 d = ld4(src + cq_q4(nxx, ix, (z0 >> 2) + q));
-        buf[(2 * q) * LS + ix] = make_float2(d.x, d.y);
-        buf[(2 * q + 1) * LS + ix] = make_float2(d.z, d.w);
-    }
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
+    cq_loop<4>(
+        nxx * Q, [&](int e) { return ld4(src + (size_t)(z0 >> 2) * nxx * 4 + (size_t)e * 4); },
+        [&](int e, float4 d) {
+            const int q = cq_div(e, inv), ix = e - q * nxx;
+            buf[(2 * q) * LS + ix] = make_float2(d.x, d.y);
+            buf[(2 * q + 1) * LS + ix] = make_float2(d.z, d.w);
+        });
 }
 
 // x-CPML on dp/dx (in LDS) and the vx update; leaves vx_new in LDS
@@ -52,33 +88,46 @@ template <int Q>
 __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 {
     const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
-    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int q = e / nxx, ix = e - q * nxx;
-        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-        const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
-        float4 dp = make_float4(a.x, a.y, b.x, b.y);
-        const float4 vx = ld4(J.vx + g), bx = ld4(J.dtbx + g);
-        if (ix < pml || ix >= nxx - pml) {
-            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float CQ_GLOBAL *php = J.phix + cq_q4(2 * pml, sx, (z0 >> 2) + q);
-            const float hb = J.hbx[ix], ha = J.hax[ix];
-            float4 ph = ld4(php);
-            ph.x = hb * ph.x + ha * dp.x;
-            ph.y = hb * ph.y + ha * dp.y;
-            ph.z = hb * ph.z + ha * dp.z;
-            ph.w = hb * ph.w + ha * dp.w;
-            st4(php, ph);
-            dp.x += ph.x; dp.y += ph.y; dp.z += ph.z; dp.w += ph.w;
-        }
-        float4 vn;
-        vn.x = vx.x + bx.x * dp.x;
-        vn.y = vx.y + bx.y * dp.y;
-        vn.z = vx.z + bx.z * dp.z;
-        vn.w = vx.w + bx.w * dp.w;
-        st4(J.vx + g, vn);
-        buf[(2 * q) * LS + ix] = make_float2(vn.x, vn.y);
-        buf[(2 * q + 1) * LS + ix] = make_float2(vn.z, vn.w);
-    }
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
+    const size_t g0 = (size_t)(z0 >> 2) * nxx * 4;
+    cq_loop<4>(
+        nxx * Q,
+        [&](int e) {
+            const int q = cq_div(e, inv), ix = e - q * nxx;
+            F4x3 r;
+            r.a = ld4(J.vx + g0 + (size_t)e * 4);
+            r.b = ld4(J.dtbx + g0 + (size_t)e * 4);
+            if (ix < pml || ix >= nxx - pml) {
+                const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+                r.c = ld4(J.phix + cq_q4(2 * pml, sx, (z0 >> 2) + q));
+            }
+            return r;
+        },
+        [&](int e, const F4x3 &r) {
+            const int q = cq_div(e, inv), ix = e - q * nxx;
+            const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
+            float4 dp = make_float4(a.x, a.y, b.x, b.y);
+            const float4 vx = r.a, bx = r.b;
+            if (ix < pml || ix >= nxx - pml) {
+                const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+                const float hb = J.hbx[ix], ha = J.hax[ix];
+                float4 ph = r.c;
+                ph.x = hb * ph.x + ha * dp.x;
+                ph.y = hb * ph.y + ha * dp.y;
+                ph.z = hb * ph.z + ha * dp.z;
+                ph.w = hb * ph.w + ha * dp.w;
+                st4(J.phix + cq_q4(2 * pml, sx, (z0 >> 2) + q), ph);
+                dp.x += ph.x; dp.y += ph.y; dp.z += ph.z; dp.w += ph.w;
+            }
+            float4 vn;
+            vn.x = vx.x + bx.x * dp.x;
+            vn.y = vx.y + bx.y * dp.y;
+            vn.z = vx.z + bx.z * dp.z;
+            vn.w = vx.w + bx.w * dp.w;
+            st4(J.vx + g0 + (size_t)e * 4, vn);
+            buf[(2 * q) * LS + ix] = make_float2(vn.x, vn.y);
+            buf[(2 * q + 1) * LS + ix] = make_float2(vn.z, vn.w);
+        });
 }
 
 // dvx/dx (in LDS) with its x-CPML memory variable -> dvx for the next step
@@ -88,31 +137,43 @@ template <int Q, bool TO_DIV = false>
 __device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, int z0)
 {
     const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
-    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int q = e / nxx, ix = e - q * nxx;
-        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
-        const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
-        float4 dv = make_float4(a.x, a.y, b.x, b.y);
-        if (ix < pml || ix >= nxx - pml) {
-            const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
-            float CQ_GLOBAL *psp = J.psix + cq_q4(2 * pml, sx, (z0 >> 2) + q);
-            const float ib = J.ibx[ix], ia = J.iax[ix];
-            float4 ps = ld4(psp);
-            ps.x = ib * ps.x + ia * dv.x;
-            ps.y = ib * ps.y + ia * dv.y;
-            ps.z = ib * ps.z + ia * dv.z;
-            ps.w = ib * ps.w + ia * dv.w;
-            st4(psp, ps);
-            dv.x += ps.x; dv.y += ps.y; dv.z += ps.z; dv.w += ps.w;
-        }
-        if (TO_DIV) {
-            const float4 dz = ld4(J.dvz + g);
-            dv.x += dz.x; dv.y += dz.y; dv.z += dz.z; dv.w += dz.w;
-            st4(J.div + g, dv);
-        } else {
-            st4(J.dvx + g, dv);
-        }
-    }
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
+    const size_t g0 = (size_t)(z0 >> 2) * nxx * 4;
+    cq_loop<4>(
+        nxx * Q,
+        [&](int e) {
+            const int q = cq_div(e, inv), ix = e - q * nxx;
+            F4x2 r;
+            if (TO_DIV) r.b = ld4(J.dvz + g0 + (size_t)e * 4);
+            if (ix < pml || ix >= nxx - pml) {
+                const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+                r.a = ld4(J.psix + cq_q4(2 * pml, sx, (z0 >> 2) + q));
+            }
+            return r;
+        },
+        [&](int e, const F4x2 &r) {
+            const int q = cq_div(e, inv), ix = e - q * nxx;
+            const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
+            float4 dv = make_float4(a.x, a.y, b.x, b.y);
+            if (ix < pml || ix >= nxx - pml) {
+                const int sx = ix < pml ? ix : ix - (nxx - 2 * pml);
+                const float ib = J.ibx[ix], ia = J.iax[ix];
+                float4 ps = r.a;
+                ps.x = ib * ps.x + ia * dv.x;
+                ps.y = ib * ps.y + ia * dv.y;
+                ps.z = ib * ps.z + ia * dv.z;
+                ps.w = ib * ps.w + ia * dv.w;
+                st4(J.psix + cq_q4(2 * pml, sx, (z0 >> 2) + q), ps);
+                dv.x += ps.x; dv.y += ps.y; dv.z += ps.z; dv.w += ps.w;
+            }
+            if (TO_DIV) {
+                const float4 dz = r.b;
+                dv.x += dz.x; dv.y += dz.y; dv.z += dz.z; dv.w += dz.w;
+                st4(J.div + g0 + (size_t)e * 4, dv);
+            } else {
+                st4(J.dvx + g0 + (size_t)e * 4, dv);
+            }
+        });
 }
 
 // ===========================================================================
@@ -127,15 +188,21 @@ template <int CW>
 __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float CQ_GLOBAL *__restrict__ src, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2;
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ix < J.nxx) v = ld4(src + cq_q4(J.nxx, ix, zq));
-        float *line = col_line(bufs, nzz, c);
+    cq_loop<4>(
+        CW * nq,
+        [&](int e) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ix < J.nxx) v = ld4(src + cq_q4(J.nxx, ix, zq));
+            return v;
+        },
+        [&](int e, float4 v) {
+            const int zq = e / CW, c = e - zq * CW;
+            float *line = col_line(bufs, nzz, c);
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
-    }
+            for (int k = 0; k < 4; k++)
+                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
+        });
 }
 
 // gather rec[t][i] from a column tile held as plain floats (src(c, iz))
@@ -154,31 +221,48 @@ template <int CW>
 __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        if (ix >= J.nxx) continue;
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        float *line = col_line(bufs, nzz, c);
-        const float4 vz = ld4(J.vz + g), bz = ld4(J.dtbz + g);
-        float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+    cq_loop<4>(
+        CW * nq,
+        [&](int e) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            F4x3 r;
+            if (ix >= J.nxx) return r;
+            const size_t g = cq_q4(J.nxx, ix, zq);
+            r.a = ld4(J.vz + g);
+            r.b = ld4(J.dtbz + g);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            float dp = line[2 * iz];
-            if (iz < pml || iz >= nzz - pml) {
-                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float CQ_GLOBAL *php = J.phiz + (size_t)sz * J.nxx + ix;
-                const float ph = J.hbz[iz] * (*php) + J.haz[iz] * dp;
-                *php = ph;
-                dp += ph;
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz < nzz && (iz < pml || iz >= nzz - pml)) {
+                    const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                    f4set(r.c, k, J.phiz[(size_t)sz * J.nxx + ix]);
+                }
             }
-            const float v = f4get(vz, k) + f4get(bz, k) * dp;
-            f4set(vn, k, v);
-            line[2 * iz] = v;
-        }
-        st4(J.vz + g, vn);
-    }
+            return r;
+        },
+        [&](int e, const F4x3 &r) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            if (ix >= J.nxx) return;
+            const size_t g = cq_q4(J.nxx, ix, zq);
+            float *line = col_line(bufs, nzz, c);
+            float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz >= nzz) continue;
+                float dp = line[2 * iz];
+                if (iz < pml || iz >= nzz - pml) {
+                    const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                    const float ph = J.hbz[iz] * f4get(r.c, k) + J.haz[iz] * dp;
+                    J.phiz[(size_t)sz * J.nxx + ix] = ph;
+                    dp += ph;
+                }
+                const float v = f4get(r.a, k) + f4get(r.b, k) * dp;
+                f4set(vn, k, v);
+                line[2 * iz] = v;
+            }
+            st4(J.vz + g, vn);
+        });
 }
 
 // dvz/dz with z-CPML, div = dvx + dvz (update_vxx_vzz + sumpxpz,
@@ -188,36 +272,51 @@ template <int CW, bool TO_DVZ = false>
 __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        float *line = col_line(bufs, nzz, c);
-        if (ix >= J.nxx) {
+    cq_loop<4>(
+        CW * nq,
+        [&](int e) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            F4x2 r;
+            r.a = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ix >= J.nxx) return r;
+            if (!TO_DVZ) r.a = ld4(J.dvx + cq_q4(J.nxx, ix, zq));
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
-            continue;
-        }
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        const float4 dvx = TO_DVZ ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(J.dvx + g);
-        float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            float d = line[2 * iz];
-            if (iz < pml || iz >= nzz - pml) {
-                const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
-                float CQ_GLOBAL *psp = J.psiz + (size_t)sz * J.nxx + ix;
-                const float ps = J.ibz[iz] * (*psp) + J.iaz[iz] * d;
-                *psp = ps;
-                d += ps;
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz < nzz && (iz < pml || iz >= nzz - pml)) {
+                    const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                    f4set(r.b, k, J.psiz[(size_t)sz * J.nxx + ix]);
+                }
             }
-            const float s = f4get(dvx, k) + d;
-            f4set(dv, k, s);
-            line[2 * iz] = s;
-        }
-        st4((TO_DVZ ? J.dvz : J.div) + g, dv);
-    }
+            return r;
+        },
+        [&](int e, const F4x2 &r) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            float *line = col_line(bufs, nzz, c);
+            if (ix >= J.nxx) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                return;
+            }
+            float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz >= nzz) continue;
+                float d = line[2 * iz];
+                if (iz < pml || iz >= nzz - pml) {
+                    const int sz = iz < pml ? iz : iz - (nzz - 2 * pml);
+                    const float ps = J.ibz[iz] * f4get(r.b, k) + J.iaz[iz] * d;
+                    J.psiz[(size_t)sz * J.nxx + ix] = ps;
+                    d += ps;
+                }
+                const float sm = f4get(r.a, k) + d;
+                f4set(dv, k, sm);
+                line[2 * iz] = sm;
+            }
+            st4((TO_DVZ ? J.dvz : J.div) + cq_q4(J.nxx, ix, zq), dv);
+        });
 }
 
 // After a forward z-FFT of packed columns (c0 + i c1): split into the two
@@ -470,12 +569,25 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
     // (sgpstd-qrtm/acps4.cu:1623-1659)
     const bool rcv = J.role == 1;
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
-    for (int e = threadIdx.x; e < nxx * Q; e += CQ_NT) {
-        const int q = e / nxx, ix = e - q * nxx;
-        const size_t g = cq_q4(nxx, ix, (z0 >> 2) + q);
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
+    const size_t g0 = (size_t)(z0 >> 2) * nxx * 4;
+    cq_loop<2>(
+        nxx * Q,
+        [&](int e) {
+            const size_t g = g0 + (size_t)e * 4;
+            F4x5 r;
+            r.a = ld4(J.p + g);
+            r.b = ld4(J.c1 + g);
+            r.c = ld4(J.c2 + g);
+            r.d = ld4(J.lz + g);
+            r.e = ld4(J.div + g);
+            return r;
+        },
+        [&](int e, const F4x5 &r) {
+        const int q = cq_div(e, inv), ix = e - q * nxx;
+        const size_t g = g0 + (size_t)e * 4;
         const float2 a = buf[(2 * q) * LS + ix], b = buf[(2 * q + 1) * LS + ix];
-        const float4 p = ld4(J.p + g), c1 = ld4(J.c1 + g), c2 = ld4(J.c2 + g), lz = ld4(J.lz + g),
-                     d = ld4(J.div + g);
+        const float4 p = r.a, c1 = r.b, c2 = r.c, lz = r.d, d = r.e;
         float4 pn;
         pn.x = p.x + c1.x * d.x - c2.x * (a.x + lz.x);
         pn.y = p.y + c1.y * d.y - c2.y * (a.y + lz.y);
@@ -491,7 +603,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
         }
         buf[(2 * q) * LS + ix] = make_float2(pn.x, pn.y);
         buf[(2 * q + 1) * LS + ix] = make_float2(pn.z, pn.w);
-    }
+        });
     __syncthreads();
     if (rcv) {
         // receiver wavefield: p = obs at the receiver cells (addsa_b_kernel,
