@@ -329,21 +329,33 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    for (int e = threadIdx.x; e < L * nzh; e += CQ_NT) {
-        const int l = e / nzh, k = e - l * nzh;
-        const int k2 = k ? nzz - k : 0;
-        float2 *line = buf + l * cq_line_stride(nzz);
-        const float2 zk = line[k], zm = line[k2];
-        const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-        const int ix0 = x0 + 2 * l;
-        if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
-        if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
-        if (mh != nullptr) {
-            line[k] = cmul(zk, mh[k]);
-            if (k2 != k) line[k2] = cmul(zm, mh[k2]);
-        }
-    }
+    const float inv = __builtin_amdgcn_rcpf((float)nzh);
+    cq_loop<4>(
+        L * nzh,
+        [&](int e) {
+            float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (mh != nullptr) {
+                const int k = e - cq_div(e, inv) * nzh;
+                const float2 a = mh[k], b = mh[k ? nzz - k : 0];
+                m = make_float4(a.x, a.y, b.x, b.y);
+            }
+            return m;
+        },
+        [&](int e, float4 m) {
+            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int k2 = k ? nzz - k : 0;
+            float2 *line = buf + l * cq_line_stride(nzz);
+            const float2 zk = line[k], zm = line[k2];
+            const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+            const int ix0 = x0 + 2 * l;
+            if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
+            if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
+            if (mh != nullptr) {
+                line[k] = cmul(zk, make_float2(m.x, m.y));
+                if (k2 != k) line[k2] = cmul(zm, make_float2(m.z, m.w));
+            }
+        });
 }
 
 // Inverse of col_extract: rebuild the packed full spectrum W = H0 + i H1 from
@@ -354,21 +366,29 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    for (int e = threadIdx.x; e < L * nzh; e += CQ_NT) {
-        const int l = e / nzh, k = e - l * nzh;
-        const int k2 = k ? nzz - k : 0;
-        const int ix0 = x0 + 2 * l;
-        float2 h0 = make_float2(0.f, 0.f), h1 = make_float2(0.f, 0.f);
-        if (ix0 < J.nxx) h0 = S[(size_t)ix0 * ldk + k];
-        if (ix0 + 1 < J.nxx) h1 = S[(size_t)(ix0 + 1) * ldk + k];
-        float2 *line = buf + l * cq_line_stride(nzz);
-        if (k2 == k) {
-            h0.y = 0.f;
-            h1.y = 0.f;
-        }
-        line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
-        if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
-    }
+    const float inv = __builtin_amdgcn_rcpf((float)nzh);
+    cq_loop<4>(
+        L * nzh,
+        [&](int e) {
+            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int ix0 = x0 + 2 * l;
+            float2 h0 = make_float2(0.f, 0.f), h1 = make_float2(0.f, 0.f);
+            if (ix0 < J.nxx) h0 = S[(size_t)ix0 * ldk + k];
+            if (ix0 + 1 < J.nxx) h1 = S[(size_t)(ix0 + 1) * ldk + k];
+            return make_float4(h0.x, h0.y, h1.x, h1.y);
+        },
+        [&](int e, float4 h) {
+            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int k2 = k ? nzz - k : 0;
+            float2 h0 = make_float2(h.x, h.y), h1 = make_float2(h.z, h.w);
+            float2 *line = buf + l * cq_line_stride(nzz);
+            if (k2 == k) {
+                h0.y = 0.f;
+                h1.y = 0.f;
+            }
+            line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
+            if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
+        });
 }
 
 // ===========================================================================
@@ -379,12 +399,17 @@ __device__ __forceinline__ void krow_fill(float2 *buf, const JobG &J, const floa
 {
     constexpr int H = TK / 2;
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
-    for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
-        const int ix = e / H, h = e % H;
-        const float4 v = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
-        buf[(2 * h) * LS + ix] = make_float2(v.x, v.y);
-        buf[(2 * h + 1) * LS + ix] = make_float2(v.z, v.w);
-    }
+    cq_loop<4>(
+        nxx * H,
+        [&](int e) {
+            const int ix = e / H, h = e % H;
+            return *reinterpret_cast<const float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
+        },
+        [&](int e, float4 v) {
+            const int ix = e / H, h = e % H;
+            buf[(2 * h) * LS + ix] = make_float2(v.x, v.y);
+            buf[(2 * h + 1) * LS + ix] = make_float2(v.z, v.w);
+        });
 }
 
 template <int TK>
@@ -777,65 +802,103 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        const float4 gm = ld4(J.qg + g);
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    // column c of the tile: ix = x0 + c; quads beyond nxx are skipped
+    auto cell = [&](int e, int &zq, int &c) {
+        zq = e / CW;
+        c = e - zq * CW;
+        return x0 + c < J.nxx;
+    };
+    cq_loop<4>(
+        CW * nq,
+        [&](int e) {
+            int zq, c;
+            float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (cell(e, zq, c)) gm = ld4(J.qg + cq_q4(J.nxx, x0 + c, zq));
+            return gm;
+        },
+        [&](int e, float4 gm) {
+            int zq, c;
+            if (!cell(e, zq, c)) return;
+            const float *line = col_line(bufs, nzz, c);
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(gm, k)) * line[2 * iz]);
-        }
-        st4(J.tot + g, a);
-    }
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz < nzz) f4set(a, k, (1.0f - 64.0f * f4get(gm, k)) * line[2 * iz]);
+            }
+            st4(J.tot + cq_q4(J.nxx, x0 + c, zq), a);
+        });
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s4);   // B
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        const float4 y2 = ld4(J.qy2 + g), cc = ld4(J.qcc + g);
-        float4 a = ld4(J.tot + g);
+    cq_loop<4>(
+        CW * nq,
+        [&](int e) {
+            int zq, c;
+            F4x3 r;
+            if (cell(e, zq, c)) {
+                const size_t g = cq_q4(J.nxx, x0 + c, zq);
+                r.a = ld4(J.qy2 + g);
+                r.b = ld4(J.qcc + g);
+                r.c = ld4(J.tot + g);
+            }
+            return r;
+        },
+        [&](int e, const F4x3 &r) {
+            int zq, c;
+            if (!cell(e, zq, c)) return;
+            const float *line = col_line(bufs, nzz, c);
+            const float4 y2 = r.a, cc = r.b;
+            float4 a = r.c;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * iz]));
-        }
-        st4(J.tot + g, a);
-    }
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz < nzz) f4set(a, k, f4get(cc, k) * (f4get(a, k) + 2.0f * f4get(y2, k) * line[2 * iz]));
+            }
+            st4(J.tot + cq_q4(J.nxx, x0 + c, zq), a);
+        });
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s1);   // A
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        float *line = col_line(bufs, nzz, c);
-        if (ix >= J.nxx) {
+    cq_loop<3>(
+        CW * nq,
+        [&](int e) {
+            int zq, c;
+            F4x5 r;
+            if (cell(e, zq, c)) {
+                const size_t g = cq_q4(J.nxx, x0 + c, zq);
+                r.a = ld4(J.qg + g);
+                r.b = ld4(J.qy2 + g);
+                r.c = ld4(J.div + g);
+                r.d = ld4(J.tot + g);
+            }
+            return r;
+        },
+        [&](int e, const F4x5 &r) {
+            int zq, c;
+            const bool in = cell(e, zq, c);
+            float *line = col_line(bufs, nzz, c);
+            if (!in) {
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
-            continue;
-        }
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        const float4 gm = ld4(J.qg + g), y2 = ld4(J.qy2 + g), d = ld4(J.div + g), a = ld4(J.tot + g);
-        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int k = 0; k < 4; k++)
+                    if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                return;
+            }
+            const float4 gm = r.a, y2 = r.b, d = r.c, a = r.d;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * iz] -
-                             f4get(a, k);
-            f4set(o, k, tv);
-            line[2 * iz] = tv;
-        }
-        st4(J.tot + g, o);
-    }
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz >= nzz) continue;
+                const float tv = (1.0f - 32.0f * f4get(gm, k)) * f4get(d, k) + f4get(y2, k) * line[2 * iz] -
+                                 f4get(a, k);
+                f4set(o, k, tv);
+                line[2 * iz] = tv;
+            }
+            st4(J.tot + cq_q4(J.nxx, x0 + c, zq), o);
+        });
     __syncthreads();
     cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
@@ -897,28 +960,44 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
     const bool rcv = J.role == 1;
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     const float s = J.qsign;
-    for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
-        const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-        float *line = col_line(bufs, nzz, c);
-        if (ix >= J.nxx) {
+    struct C3In {
+        float4 p, tot, qa, qb, qe, tt;
+    };
+    cq_loop<2>(
+        CW * nq,
+        [&](int e) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            C3In r;
+            if (ix < J.nxx) {
+                const size_t g = cq_q4(J.nxx, ix, zq);
+                r.p = ld4(J.p + g);
+                r.tot = ld4(J.tot + g);
+                r.qa = ld4(J.qa + g);
+                r.qb = ld4(J.qb + g);
+                r.qe = ld4(J.qe + g);
+                r.tt = ld4(J.scr + g);
+            }
+            return r;
+        },
+        [&](int e, const C3In &r) {
+            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            float *line = col_line(bufs, nzz, c);
+            if (ix >= J.nxx) {
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
-            continue;
-        }
-        const size_t g = cq_q4(J.nxx, ix, zq);
-        const float4 p = ld4(J.p + g), tot = ld4(J.tot + g), qa = ld4(J.qa + g), qb = ld4(J.qb + g),
-                     qe = ld4(J.qe + g), tt = ld4(J.scr + g);
+                for (int k = 0; k < 4; k++)
+                    if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                return;
+            }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int iz = 4 * zq + k;
-            if (iz >= nzz) continue;
-            float v = f4get(p, k) + f4get(qa, k) * (f4get(tot, k) - s * f4get(qb, k) * f4get(tt, k)) -
-                      s * f4get(qe, k) * line[2 * iz];
-            if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
-            line[2 * iz] = v;
-        }
-    }
+            for (int k = 0; k < 4; k++) {
+                const int iz = 4 * zq + k;
+                if (iz >= nzz) continue;
+                float v = f4get(r.p, k) + f4get(r.qa, k) * (f4get(r.tot, k) - s * f4get(r.qb, k) * f4get(r.tt, k)) -
+                          s * f4get(r.qe, k) * line[2 * iz];
+                if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
+                line[2 * iz] = v;
+            }
+        });
     __syncthreads();
     if (rcv) {
         // receiver wavefield: p = obs at the receiver cells (addsa_b_kernel,
