@@ -330,6 +330,13 @@ struct cq_batch_s {
     bool timing = false;
     std::vector<hipEvent_t> ev;            // start of step, then one per launch
     std::vector<int> ev_pass;
+    // The acoustic passes run on a second stream while the viscoacoustic
+    // passes run on the context's stream: the two chains touch different
+    // jobs, so their kernels may overlap on the device.  `split` is set
+    // while the side stream has forked from the context stream.
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool split = false;
     ~cq_batch_s()
     {
         for (auto &j : jobs) (void)hipFree(j.mem);
@@ -337,6 +344,9 @@ struct cq_batch_s {
         for (int r = 0; r < 2; r++)
             for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
         for (auto e : ev) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
@@ -895,7 +905,8 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
     const size_t lds = b->lds[pass];
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, b->ctx->stream, b->djobs,
+    hipStream_t st = b->split && (pass == P_AROW || pass == P_ACOL) ? b->side : b->ctx->stream;
+    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, st, b->djobs,
                        b->ditems[role][pass], step);
     HIPCHK(hipGetLastError());
     return CQ_OK;
@@ -997,23 +1008,70 @@ static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
 
 // Run global steps b->t .. end - 1; before each, the checkpoint copies that
 // the RTM shots' schedules place there.
-static int advance_to(cq_batch b, int end)
+// Start (fork) or end (join) the side stream's chain of acoustic passes.
+// Forking only pays when both kinds have work; per-launch timing keeps one
+// stream so that its events bracket single kernels.
+static int fork_side(cq_batch b)
+{
+    bool ac = false, q = false;
+    for (int r = 0; r < 2; r++) {
+        ac = ac || b->nitems[r][P_AROW] > 0;
+        q = q || b->nitems[r][P_QC1] > 0;
+    }
+    if (b->timing || !ac || !q) return CQ_OK;
+    if (!b->side) {
+        HIPCHK(hipStreamCreate(&b->side));
+        HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(b->fork_ev, b->ctx->stream));
+    HIPCHK(hipStreamWaitEvent(b->side, b->fork_ev, 0));
+    b->split = true;
+    return CQ_OK;
+}
+static int join_side(cq_batch b)
+{
+    if (!b->split) return CQ_OK;
+    b->split = false;
+    HIPCHK(hipEventRecord(b->join_ev, b->side));
+    HIPCHK(hipStreamWaitEvent(b->ctx->stream, b->join_ev, 0));
+    return CQ_OK;
+}
+
+static int advance_steps(cq_batch b, int end)
 {
     for (; b->t < end; b->t++) {
         for (auto &h : b->jobs) {
             const int s = b->t - h.t0;
             while (h.op_next < h.ops.size() && h.ops[h.op_next].at <= s) {
                 if (h.ops[h.op_next].at == s) {
-                    int rc = ckpt_apply(b, h, h.ops[h.op_next]);
+                    // checkpoint copies run on the context stream, after
+                    // every earlier pass of either chain
+                    int rc = join_side(b);
+                    if (rc) return rc;
+                    rc = ckpt_apply(b, h, h.ops[h.op_next]);
                     if (rc) return rc;
                 }
                 h.op_next++;
             }
         }
+        if (!b->split) {
+            int rc = fork_side(b);
+            if (rc) return rc;
+        }
         int rc = one_step(b, b->t);
         if (rc) return rc;
     }
     return CQ_OK;
+}
+
+// Global steps b->t .. end - 1; on return all work is ordered on the
+// context stream.
+static int advance_to(cq_batch b, int end)
+{
+    const int rc = advance_steps(b, end);
+    const int rj = join_side(b);
+    return rc ? rc : rj;
 }
 
 extern "C" int cq_batch_step(cq_batch b, int nsteps)
