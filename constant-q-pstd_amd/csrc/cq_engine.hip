@@ -306,6 +306,7 @@ struct JobH {
 
 // One launch of the per-step sequence.
 enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_QC2, P_QR2, P_QC3, P_QR3, P_COUNT };
+static const int kGroups = 4;   // launch groups of a batch (see cq_batch_s::goff)
 static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1",
                                          "k_q_c2",   "k_q_r2",   "k_q_c3", "k_q_r3"};
 
@@ -330,12 +331,15 @@ struct cq_batch_s {
     bool timing = false;
     std::vector<hipEvent_t> ev;            // start of step, then one per launch
     std::vector<int> ev_pass;
-    // The acoustic passes run on a second stream while the viscoacoustic
-    // passes run on the context's stream: the two chains touch different
-    // jobs, so their kernels may overlap on the device.  `split` is set
-    // while the side stream has forked from the context stream.
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // Item groups: the jobs of a batch are split into kGroups groups (two of
+    // viscoacoustic jobs, two of acoustic jobs), each with its own chain of
+    // launches on its own stream (group 0 on the context's stream), so that
+    // kernels of different groups overlap on the device.  Group g's items of
+    // a pass are ditems[r][p][goff[r][p][g] .. goff[r][p][g + 1]).  `split`
+    // is set while the side streams have forked from the context stream.
+    int goff[2][P_COUNT][kGroups + 1] = {};
+    hipStream_t side[kGroups] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kGroups] = {};
     bool split = false;
     ~cq_batch_s()
     {
@@ -345,8 +349,10 @@ struct cq_batch_s {
             for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
         for (auto e : ev) (void)hipEventDestroy(e);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
-        if (join_ev) (void)hipEventDestroy(join_ev);
-        if (side) (void)hipStreamDestroy(side);
+        for (int g = 1; g < kGroups; g++) {
+            if (join_ev[g]) (void)hipEventDestroy(join_ev[g]);
+            if (side[g]) (void)hipStreamDestroy(side[g]);
+        }
     }
 };
 
@@ -826,16 +832,18 @@ static int rebuild_tables(cq_batch b)
     const int cap = (int)b->djobs_cap;
     std::vector<JobD> hd(2 * (size_t)cap);
     memset(hd.data(), 0, sizeof(JobD) * hd.size());
-    std::vector<int2> items[2][P_COUNT];
+    std::vector<int2> items[kGroups][2][P_COUNT];
     b->tmax = 0;
+    int nq = 0, na = 0;   // jobs of each kind so far (round-robin over the kind's two groups)
     for (size_t i = 0; i < n; i++) {
         const JobH &h = b->jobs[i];
         hd[i] = h.d[0];
         hd[cap + i] = h.nroles > 1 ? h.d[1] : h.d[0];
         b->tmax = std::max(b->tmax, h.t0 + h.nsched);
+        const int g = is_q(h.kind) ? (nq++ & 1) : 2 + (na++ & 1);
         for (int r = 0; r < h.nroles; r++) {
             const int e = r ? cap + (int)i : (int)i;
-            std::vector<int2> *it = items[r];
+            std::vector<int2> *it = items[g][r];
             if (!is_q(h.kind)) {
                 for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_AROW].push_back(make_int2(e, z0));
                 for (int x0 = 0; x0 < h.nxx; x0 += b->cw) it[P_ACOL].push_back(make_int2(e, x0));
@@ -856,8 +864,15 @@ static int rebuild_tables(cq_batch b)
     for (int r = 0; r < 2; r++)
         for (int p = 0; p < P_COUNT; p++) {
             b->nitems[r][p] = 0;
-            if (items[r][p].empty()) continue;
-            ord[r][p] = xcd_order(items[r][p]);
+            // groups one after another, each in XCD order
+            for (int g = 0; g < kGroups; g++) {
+                b->goff[r][p][g] = (int)ord[r][p].size();
+                if (items[g][r][p].empty()) continue;
+                const std::vector<int2> o = xcd_order(items[g][r][p]);
+                ord[r][p].insert(ord[r][p].end(), o.begin(), o.end());
+            }
+            b->goff[r][p][kGroups] = (int)ord[r][p].size();
+            if (ord[r][p].empty()) continue;
             if (ord[r][p].size() > b->ditems_cap[r][p]) {
                 (void)hipFree(b->ditems[r][p]);
                 b->ditems[r][p] = nullptr;
@@ -905,10 +920,17 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
     const size_t lds = b->lds[pass];
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipStream_t st = b->split && (pass == P_AROW || pass == P_ACOL) ? b->side : b->ctx->stream;
-    hipLaunchKernelGGL(kernel, dim3(b->nitems[role][pass]), dim3(CQ_NT), lds, st, b->djobs,
-                       b->ditems[role][pass], step);
-    HIPCHK(hipGetLastError());
+    // one launch per group on its stream, or, unsplit, one for all groups
+    for (int g = 0; g < kGroups; g++) {
+        const int o = b->split ? b->goff[role][pass][g] : 0;
+        const int cnt = b->split ? b->goff[role][pass][g + 1] - o : b->nitems[role][pass];
+        if (cnt > 0) {
+            hipStream_t st = b->split && g > 0 ? b->side[g] : b->ctx->stream;
+            hipLaunchKernelGGL(kernel, dim3(cnt), dim3(CQ_NT), lds, st, b->djobs, b->ditems[role][pass] + o, step);
+            HIPCHK(hipGetLastError());
+        }
+        if (!b->split) break;
+    }
     return CQ_OK;
 }
 
@@ -1008,24 +1030,21 @@ static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
 
 // Run global steps b->t .. end - 1; before each, the checkpoint copies that
 // the RTM shots' schedules place there.
-// Start (fork) or end (join) the side stream's chain of acoustic passes.
-// Forking only pays when both kinds have work; per-launch timing keeps one
-// stream so that its events bracket single kernels.
+// Start (fork) or end (join) the side streams of launch groups 1 ..
+// kGroups - 1.  Per-launch timing keeps one stream so that its events
+// bracket single passes.
 static int fork_side(cq_batch b)
 {
-    bool ac = false, q = false;
-    for (int r = 0; r < 2; r++) {
-        ac = ac || b->nitems[r][P_AROW] > 0;
-        q = q || b->nitems[r][P_QC1] > 0;
-    }
-    if (b->timing || !ac || !q) return CQ_OK;
-    if (!b->side) {
-        HIPCHK(hipStreamCreate(&b->side));
+    if (b->timing) return CQ_OK;
+    if (!b->fork_ev) {
         HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+        for (int g = 1; g < kGroups; g++) {
+            HIPCHK(hipStreamCreate(&b->side[g]));
+            HIPCHK(hipEventCreateWithFlags(&b->join_ev[g], hipEventDisableTiming));
+        }
     }
     HIPCHK(hipEventRecord(b->fork_ev, b->ctx->stream));
-    HIPCHK(hipStreamWaitEvent(b->side, b->fork_ev, 0));
+    for (int g = 1; g < kGroups; g++) HIPCHK(hipStreamWaitEvent(b->side[g], b->fork_ev, 0));
     b->split = true;
     return CQ_OK;
 }
@@ -1033,8 +1052,10 @@ static int join_side(cq_batch b)
 {
     if (!b->split) return CQ_OK;
     b->split = false;
-    HIPCHK(hipEventRecord(b->join_ev, b->side));
-    HIPCHK(hipStreamWaitEvent(b->ctx->stream, b->join_ev, 0));
+    for (int g = 1; g < kGroups; g++) {
+        HIPCHK(hipEventRecord(b->join_ev[g], b->side[g]));
+        HIPCHK(hipStreamWaitEvent(b->ctx->stream, b->join_ev[g], 0));
+    }
     return CQ_OK;
 }
 
