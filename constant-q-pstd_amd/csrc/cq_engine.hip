@@ -212,7 +212,7 @@ static StateLayout state_layout(int nxx, int ldz, int ldk, int pml, bool q)
     s.vz = A.take<float>(N);
     s.div = A.take<float>(N);
     s.x = A.take<float>(N);
-    s.y = q ? A.take<float2>((size_t)nxx * ldk) : A.take<float>(N);
+    s.y = q ? A.take<float2>((size_t)cq_hs_cols(nxx) * ldk) : A.take<float>(N);
     s.psix = A.take<float>((size_t)2 * pml * ldz);
     s.phix = A.take<float>((size_t)2 * pml * ldz);
     s.psiz = A.take<float>((size_t)2 * pml * nxx);
@@ -489,9 +489,9 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         for (int i = 0; i < 4; i++) o_q[i] = A.take<float>(N);
         o_tot = A.take<float>(N);
         o_scr = A.take<float>(N);
-        o_s1 = A.take<float2>((size_t)nxx * ldk);
-        o_s3 = A.take<float2>((size_t)nxx * ldk);
-        o_s4 = A.take<float2>((size_t)nxx * ldk);
+        o_s1 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
+        o_s3 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
+        o_s4 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
     }
     const size_t o_1x = A.take<float>((size_t)4 * nxx), o_1z = A.take<float>((size_t)4 * nzz);
     const size_t o_mx = A.take<float2>((size_t)3 * nxx), o_mz = A.take<float2>((size_t)3 * nzz);

@@ -11,8 +11,14 @@
 // contiguous run per band) and a column tile reads CW consecutive quads per
 // band.  ldz (nzz rounded up to whole row tiles and 32) depths are stored;
 // the padding depths are never read as data.
-// Half spectra in kz (viscoacoustic path): float2 [nxx][ldk], kz in [0, nzh).
 __host__ __device__ inline size_t cq_q4(int nxx, int ix, int zq) { return ((size_t)zq * nxx + ix) * 4; }
+// Half spectra in kz (viscoacoustic path): float2, kz in [0, nzh), columns in
+// blocks of 8: (ix, kz) at cq_hs(ldk, ix, kz).  A kz-row tile reads one
+// contiguous run of TK x 8 values per block; a column tile reads, per kz,
+// one 64-byte run per block it covers.  nxx is rounded up to whole blocks
+// (cq_hs_cols); the extra columns are never read as data.
+__host__ __device__ inline size_t cq_hs(int ldk, int ix, int kz) { return ((size_t)(ix >> 3) * ldk + kz) * 8 + (ix & 7); }
+__host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
 //
 // The fields of a job record.  G qualifies the device pointers: empty in the
 // host's JobD, the global address space in the kernels' JobG view of the same

@@ -329,28 +329,30 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    const float inv = __builtin_amdgcn_rcpf((float)nzh);
+    // lanes run over the column pairs of one kz, then the next kz
     cq_loop<4>(
         L * nzh,
         [&](int e) {
             float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
             if (mh != nullptr) {
-                const int k = e - cq_div(e, inv) * nzh;
+                const int k = e / L;
                 const float2 a = mh[k], b = mh[k ? nzz - k : 0];
                 m = make_float4(a.x, a.y, b.x, b.y);
             }
             return m;
         },
         [&](int e, float4 m) {
-            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int k = e / L, l = e - k * L;
             const int k2 = k ? nzz - k : 0;
             float2 *line = buf + l * cq_line_stride(nzz);
             const float2 zk = line[k], zm = line[k2];
             const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
             const int ix0 = x0 + 2 * l;
-            if (ix0 < J.nxx) S[(size_t)ix0 * ldk + k] = g0;
-            if (ix0 + 1 < J.nxx) S[(size_t)(ix0 + 1) * ldk + k] = g1;
+            // columns ix0, ix0 + 1 share a block (ix0 even); with odd nxx the
+            // last pair's second column is a block's spare column
+            if (ix0 < J.nxx)
+                *reinterpret_cast<float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k)) = make_float4(g0.x, g0.y, g1.x, g1.y);
             if (mh != nullptr) {
                 line[k] = cmul(zk, make_float2(m.x, m.y));
                 if (k2 != k) line[k2] = cmul(zm, make_float2(m.z, m.w));
@@ -359,26 +361,26 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
 }
 
 // Inverse of col_extract: rebuild the packed full spectrum W = H0 + i H1 from
-// two Hermitian half spectra (imaginary parts dropped at the self-conjugate
-// bins k = 0 and k = nzz/2, as a complex-to-real transform does).
+// the two half spectra (W[k] = H0[k] + i H1[k], W[-k] = conj H0[k] + i conj
+// H1[k]; the imaginary parts at k = 0 and k = nzz/2 are dropped, as a real
+// column's spectrum has none).
 template <int CW>
 __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, const float2 CQ_GLOBAL *__restrict__ S)
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    const float inv = __builtin_amdgcn_rcpf((float)nzh);
     cq_loop<4>(
         L * nzh,
         [&](int e) {
-            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int k = e / L, l = e - k * L;
             const int ix0 = x0 + 2 * l;
-            float2 h0 = make_float2(0.f, 0.f), h1 = make_float2(0.f, 0.f);
-            if (ix0 < J.nxx) h0 = S[(size_t)ix0 * ldk + k];
-            if (ix0 + 1 < J.nxx) h1 = S[(size_t)(ix0 + 1) * ldk + k];
-            return make_float4(h0.x, h0.y, h1.x, h1.y);
+            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
+            if (ix0 + 1 >= J.nxx) h.z = h.w = 0.f;
+            return h;
         },
         [&](int e, float4 h) {
-            const int l = cq_div(e, inv), k = e - l * nzh;
+            const int k = e / L, l = e - k * L;
             const int k2 = k ? nzz - k : 0;
             float2 h0 = make_float2(h.x, h.y), h1 = make_float2(h.z, h.w);
             float2 *line = buf + l * cq_line_stride(nzz);
@@ -394,33 +396,44 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
 // ===========================================================================
 // kz-row helpers (half spectra, TK complex lines of length nxx)
 // ===========================================================================
+// kz rows k0 .. k0 + TK - 1 of a half spectrum: per block of 8 columns one
+// contiguous run of TK x 8 values, read or written as 4 TK float4 pieces
+// (piece w: row w / 4, columns 2 (w % 4), + 1 of the block).
 template <int TK>
 __device__ __forceinline__ void krow_fill(float2 *buf, const JobG &J, const float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
-    constexpr int H = TK / 2;
+    constexpr int P = 4 * TK;
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
     cq_loop<4>(
-        nxx * H,
+        ((nxx + 7) >> 3) * P,
         [&](int e) {
-            const int ix = e / H, h = e % H;
-            return *reinterpret_cast<const float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h);
+            const int b = e / P, w = e - b * P;
+            return *reinterpret_cast<const float4 CQ_GLOBAL *>(S + ((size_t)b * J.ldk + k0 + (w >> 2)) * 8 +
+                                                                2 * (w & 3));
         },
         [&](int e, float4 v) {
-            const int ix = e / H, h = e % H;
-            buf[(2 * h) * LS + ix] = make_float2(v.x, v.y);
-            buf[(2 * h + 1) * LS + ix] = make_float2(v.z, v.w);
+            const int b = e / P, w = e - b * P;
+            const int ix = 8 * b + 2 * (w & 3);
+            float2 *line = buf + (w >> 2) * LS;
+            if (ix < nxx) line[ix] = make_float2(v.x, v.y);
+            if (ix + 1 < nxx) line[ix + 1] = make_float2(v.z, v.w);
         });
 }
 
 template <int TK>
 __device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
-    constexpr int H = TK / 2;
+    constexpr int P = 4 * TK;
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
-    for (int e = threadIdx.x; e < nxx * H; e += CQ_NT) {
-        const int ix = e / H, h = e % H;
-        const float2 a = buf[(2 * h) * LS + ix], b = buf[(2 * h + 1) * LS + ix];
-        *reinterpret_cast<float4 CQ_GLOBAL *>(S + (size_t)ix * J.ldk + k0 + 2 * h) = make_float4(a.x, a.y, b.x, b.y);
+    const int total = ((nxx + 7) >> 3) * P;
+    for (int e = threadIdx.x; e < total; e += CQ_NT) {
+        const int b = e / P, w = e - b * P;
+        const int ix = 8 * b + 2 * (w & 3);
+        const float2 *line = buf + (w >> 2) * LS;
+        const float2 a = ix < nxx ? line[ix] : make_float2(0.f, 0.f);
+        const float2 c = ix + 1 < nxx ? line[ix + 1] : make_float2(0.f, 0.f);
+        *reinterpret_cast<float4 CQ_GLOBAL *>(S + ((size_t)b * J.ldk + k0 + (w >> 2)) * 8 + 2 * (w & 3)) =
+            make_float4(a.x, a.y, c.x, c.y);
     }
 }
 
