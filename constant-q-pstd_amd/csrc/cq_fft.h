@@ -19,6 +19,9 @@
 #define CQ_NT 512
 #endif
 #define CQ_MAXST 12      // max stages per plan
+// Occupancy every pass is compiled for: two 512-thread workgroups per CU,
+// i.e. 4 waves per SIMD, which caps a kernel at 128 VGPRs.
+#define CQ_WAVES_PER_SIMD 4
 
 // butterflies per thread per stage for radix R
 __host__ __device__ constexpr int cq_nb(int R)

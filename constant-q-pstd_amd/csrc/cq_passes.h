@@ -34,6 +34,7 @@ template <int U, typename Pre, typename Post>
 __device__ __forceinline__ void cq_loop(int total, Pre pre, Post post)
 {
     using T = decltype(pre(0));
+#pragma unroll 1
     for (int e0 = threadIdx.x; e0 < total; e0 += U * CQ_NT) {
         T d[U];
 #pragma unroll
@@ -393,6 +394,87 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
         });
 }
 
+// Packed full spectrum of two real columns from their half spectra h0, h1
+// at kz k (and -k = k2): W[k] = h0 + i h1, W[-k] = conj h0 + i conj h1; the
+// imaginary parts at k = 0 and k = nzz/2 are dropped, as a real column's
+// spectrum has none.
+__device__ __forceinline__ void pack_pair(float2 *line, int k, int k2, float2 h0, float2 h1)
+{
+    if (k2 == k) {
+        h0.y = 0.f;
+        h1.y = 0.f;
+    }
+    line[k] = make_float2(h0.x - h1.y, h0.y + h1.x);
+    if (k2 != k) line[k2] = make_float2(h0.x + h1.y, h1.x - h0.y);
+}
+
+// NC columns x0 .. x0 + NC - 1, one per line (line j at buf + j LS): the
+// packed spectrum of the two real signals Sa and Sb of column x0 + j
+// (inverse-transformed, Sa lands in the real and Sb in the imaginary part).
+template <int NC>
+__device__ __forceinline__ void col2_insert(const JobG &J, float2 *buf, int LS, int x0,
+                                            const float2 CQ_GLOBAL *__restrict__ Sa,
+                                            const float2 CQ_GLOBAL *__restrict__ Sb)
+{
+    const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
+    const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NC));   // opaque: see k_q_c2
+    cq_loop<4>(
+        NC * nzh,
+        [&](int e) {
+            const int k = cq_div(e, inv), ix = x0 + (e - k * NC);
+            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ix < J.nxx) {
+                const float2 a = Sa[cq_hs(ldk, ix, k)], b = Sb[cq_hs(ldk, ix, k)];
+                h = make_float4(a.x, a.y, b.x, b.y);
+            }
+            return h;
+        },
+        [&](int e, float4 h) {
+            const int k = cq_div(e, inv), j = e - k * NC;
+            pack_pair(buf + j * LS, k, k ? nzz - k : 0, make_float2(h.x, h.y), make_float2(h.z, h.w));
+        });
+}
+
+// NP column pairs x0 + 2 l, + 1 (line l at buf + l LS), two columns per
+// line: col_insert / col_extract on explicit lines.
+template <int NP>
+__device__ __forceinline__ void colpair_insert(const JobG &J, float2 *buf, int LS, int x0,
+                                               const float2 CQ_GLOBAL *__restrict__ S)
+{
+    const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
+    const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NP));
+    cq_loop<4>(
+        NP * nzh,
+        [&](int e) {
+            const int k = cq_div(e, inv), ix0 = x0 + 2 * (e - k * NP);
+            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
+            if (ix0 + 1 >= J.nxx) h.z = h.w = 0.f;
+            return h;
+        },
+        [&](int e, float4 h) {
+            const int k = cq_div(e, inv), l = e - k * NP;
+            pack_pair(buf + l * LS, k, k ? nzz - k : 0, make_float2(h.x, h.y), make_float2(h.z, h.w));
+        });
+}
+template <int NP>
+__device__ __forceinline__ void colpair_extract(const JobG &J, const float2 *buf, int LS, int x0,
+                                                float2 CQ_GLOBAL *__restrict__ S)
+{
+    const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
+    const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NP));
+    for (int e = threadIdx.x; e < NP * nzh; e += CQ_NT) {
+        const int k = cq_div(e, inv), l = e - k * NP;
+        const int k2 = k ? nzz - k : 0;
+        const float2 *line = buf + l * LS;
+        const float2 zk = line[k], zm = line[k2];
+        const int ix0 = x0 + 2 * l;
+        if (ix0 < J.nxx)
+            *reinterpret_cast<float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k)) =
+                make_float4(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y), 0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+}
+
 // ===========================================================================
 // kz-row helpers (half spectra, TK complex lines of length nxx)
 // ===========================================================================
@@ -588,7 +670,7 @@ __device__ __forceinline__ void interior_accum(const JobG &J, float CQ_GLOBAL *d
 
 // row pass: L = Lx + Lz, p update + source, dp/dx -> vx, dvx/dx -> dvx
 template <int TZ>
-__global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                   int step)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
@@ -674,7 +756,7 @@ __global__ __launch_bounds__(CQ_NT) void k_row_ac(const JobD *__restrict__ jobs,
 
 // column pass: gather, dp/dz -> vz, dvz/dz -> div, Lz = d2/dz2 div
 template <int CW>
-__global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_col_ac(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                   int step)
 {
     constexpr int L = CW / 2;
@@ -724,7 +806,7 @@ __global__ __launch_bounds__(CQ_NT) void k_col_ac(const JobD *__restrict__ jobs,
 //   R3 (rows) : dp/dx -> vx; dvx/dx; div = dvx + dvz
 // ===========================================================================
 template <int CW>
-__global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     constexpr int L = CW / 2;
@@ -741,7 +823,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c1(const JobD *__restrict__ jobs, c
 }
 
 template <int TK>
-__global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     CQ_ITEM_PROLOGUE
@@ -798,7 +880,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r1(const JobD *__restrict__ jobs, c
 }
 
 template <int CW>
-__global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     constexpr int L = CW / 2;
@@ -810,8 +892,101 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
     load_tw(tw, J.twz, nzz);
     // total = y1 div + y2 A - (dt^2 v^2 lam/12)(y3 K + y4 B), with
     // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2.
-    // The partial sums live in `tot` itself; each thread re-reads only the
-    // elements it wrote (they stay in L2).
+    if constexpr (CW >= 24) {
+        // Two halves of NC columns (narrower tiles keep the one-symbol
+        // schedule below: their halves would transform too few lines).  K and B of one column share a line
+        // (real and imaginary part), so the partial sum
+        //   P = cc (y3 K + y4 B) - y1 div
+        // of a cell is formed in LDS; P of column pairs is then packed into
+        // the even lines, A goes into the odd lines, total = y2 A - P.
+        constexpr int NC = CW / 2, NP = NC / 2;
+        const int LS = cq_line_stride(nzz);
+        // item -> (quad, column) by the reciprocal of an opaque NC: with a
+        // compile-time power of two the compiler hoists per-thread offsets
+        // of every loop out of the half loop and runs out of registers
+        const float inv_nc = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NC));
+#pragma unroll 1
+        for (int h = 0; h < 2; h++) {
+            const int xh = x0 + h * NC;
+            if (xh >= J.nxx) break;
+            col2_insert<NC>(J, buf, LS, xh, J.s3, J.s4);   // K + i B
+            __syncthreads();
+            cq_fft_inv(J.pz, buf, LS, NC, tw);
+            cq_loop<2>(
+                NC * nq,
+                [&](int e) {
+                    const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
+                    F4x5 r;
+                    if (ix < J.nxx) {
+                        const size_t g = cq_q4(J.nxx, ix, zq);
+                        r.a = ld4(J.qg + g);
+                        r.b = ld4(J.qy2 + g);
+                        r.c = ld4(J.qcc + g);
+                        r.d = ld4(J.div + g);
+                    }
+                    return r;
+                },
+                [&](int e, const F4x5 &r) {
+                    const int zq = cq_div(e, inv_nc), j = e - zq * NC;
+                    if (xh + j >= J.nxx) return;
+                    float2 *line = buf + j * LS;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int iz = 4 * zq + k;
+                        if (iz >= nzz) continue;
+                        const float gm = f4get(r.a, k), y2 = f4get(r.b, k);
+                        const float2 kb = line[iz];
+                        line[iz].x = f4get(r.c, k) * ((1.0f - 64.0f * gm) * kb.x + 2.0f * y2 * kb.y) -
+                                     (1.0f - 32.0f * gm) * f4get(r.d, k);
+                    }
+                });
+            __syncthreads();
+            for (int e = threadIdx.x; e < NP * nzz; e += CQ_NT) {
+                const int pr = e / nzz, z = e - pr * nzz;
+                buf[2 * pr * LS + z].y = buf[(2 * pr + 1) * LS + z].x;
+            }
+            __syncthreads();
+            colpair_insert<NP>(J, buf + LS, 2 * LS, xh, J.s1);   // A
+            __syncthreads();
+            cq_fft_inv(J.pz, buf + LS, 2 * LS, NP, tw);
+            cq_loop<4>(
+                NC * nq,
+                [&](int e) {
+                    const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
+                    float4 y2 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (ix < J.nxx) y2 = ld4(J.qy2 + cq_q4(J.nxx, ix, zq));
+                    return y2;
+                },
+                [&](int e, float4 y2) {
+                    const int zq = cq_div(e, inv_nc), j = e - zq * NC, ix = xh + j;
+                    float *la = reinterpret_cast<float *>(buf + (2 * (j >> 1) + 1) * LS) + (j & 1);
+                    const float *lp = reinterpret_cast<const float *>(buf + 2 * (j >> 1) * LS) + (j & 1);
+                    if (ix >= J.nxx) {
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (4 * zq + k < nzz) la[2 * (4 * zq + k)] = 0.f;
+                        return;
+                    }
+                    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int iz = 4 * zq + k;
+                        if (iz >= nzz) continue;
+                        const float tv = f4get(y2, k) * la[2 * iz] - lp[2 * iz];
+                        f4set(o, k, tv);
+                        la[2 * iz] = tv;
+                    }
+                    st4(J.tot + cq_q4(J.nxx, ix, zq), o);
+                });
+            __syncthreads();
+            cq_fft_fwd(J.pz, buf + LS, 2 * LS, NP, tw, nullptr);
+            colpair_extract<NP>(J, buf + LS, 2 * LS, xh, J.s1);
+            __syncthreads();
+        }
+        return;
+    }
+    // Two columns per line, one symbol at a time: the partial sums live in
+    // `tot` itself; each thread re-reads only the elements it wrote.
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
     cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
@@ -918,7 +1093,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c2(const JobD *__restrict__ jobs, c
 }
 
 template <int TK>
-__global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     CQ_ITEM_PROLOGUE
@@ -936,7 +1111,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_r2(const JobD *__restrict__ jobs, c
 }
 
 template <int CW>
-__global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     constexpr int L = CW / 2;
@@ -1047,7 +1222,7 @@ __global__ __launch_bounds__(CQ_NT) void k_q_c3(const JobD *__restrict__ jobs, c
 }
 
 template <int TZ>
-__global__ __launch_bounds__(CQ_NT) void k_q_r3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r3(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
