@@ -325,7 +325,7 @@ struct cq_batch_s {
     int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
-    int tk2 = 8;                              // kz-row tile of k_q_r2 (tk or 2 tk)
+    int tk2 = 8;                              // kz-row tile of k_q_r2 (tk, 2 tk or 3 tk)
     int maxnxx = 0, maxnzz = 0;
     int t = 0, tmax = 0;                   // next global step; end of the last wavefield
     bool timing = false;
@@ -448,7 +448,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = ((nzz + tz - 1) / tz * tz + 31) & ~31;   // whole row tiles, 128-B rows
     const int nzh = nzz / 2 + 1;
-    const int ldk = (nzh + tk2 - 1) / tk2 * tk2;   // whole tiles of both kz-row passes (tk2 = tk or 2 tk)
+    const int ldk = (nzh + tk2 - 1) / tk2 * tk2;   // whole tiles of both kz-row passes (tk | tk2)
     const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
     const size_t N = (size_t)nxx * ldz;
     const int ntile_x = (nxx + cw - 1) / cw, ntile_z = (nzz + tz - 1) / tz;
@@ -804,10 +804,19 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
     b->cw = kCwCand[ic];
     b->cwq = kCwCand[icq];
     b->tk = kTkCand[ik];
-    // k_q_r2 takes two kz-row tiles at once where every length allows it
-    b->tk2 = 2 * b->tk;
-    for (int n : qxs)
-        if (!has_plan(n, b->tk2 * n) || lds_lines(n, b->tk2) > kLdsTile) b->tk2 = b->tk;
+    // k_q_r2 takes three or two kz-row tiles at once where every length
+    // allows it (more lines per transform keep more of the workgroup busy)
+    b->tk2 = b->tk;
+    for (int m : {3, 2}) {
+        const int t2 = m * b->tk;
+        if (t2 > 16) continue;
+        bool ok = true;
+        for (int n : qxs) ok = ok && has_plan(n, t2 * n) && lds_lines(n, t2) <= kLdsTile;
+        if (ok) {
+            b->tk2 = t2;
+            break;
+        }
+    }
     b->maxnxx = std::max(ax, qx);
     b->maxnzz = std::max(az, qz);
     set_lds(b, ax, az, qx, qz, qxs);
@@ -948,9 +957,11 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
      : W == 4  ? launch(b, K<4>, CQ_ARGS)                                          \
                : launch(b, K<2>, CQ_ARGS))
 #define CQ_DISPATCH_TK(K, T)                                                       \
-    (T == 16  ? launch(b, K<16>, CQ_ARGS)                                          \
-     : T == 8 ? launch(b, K<8>, CQ_ARGS)                                           \
-     : T == 4 ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
+    (T == 16   ? launch(b, K<16>, CQ_ARGS)                                         \
+     : T == 12 ? launch(b, K<12>, CQ_ARGS)                                         \
+     : T == 8  ? launch(b, K<8>, CQ_ARGS)                                          \
+     : T == 6  ? launch(b, K<6>, CQ_ARGS)                                          \
+     : T == 4  ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
 
 static int launch_pass(cq_batch b, int pass, int role, int step)
 {
@@ -958,7 +969,9 @@ static int launch_pass(cq_batch b, int pass, int role, int step)
     case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
     case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, b->cw);
     case P_QC1: return CQ_DISPATCH_CW(k_q_c1, b->cwq);
-    case P_QR1: return CQ_DISPATCH_TK(k_q_r1, b->tk);
+    case P_QR1:
+        return b->tk == 8 ? launch(b, k_q_r1<8>, CQ_ARGS)
+               : b->tk == 4 ? launch(b, k_q_r1<4>, CQ_ARGS) : launch(b, k_q_r1<2>, CQ_ARGS);
     case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
     case P_QR2: return CQ_DISPATCH_TK(k_q_r2, b->tk2);
     case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
@@ -1226,7 +1239,12 @@ static int server_tiles(cq_batch b, int mx, int mz)
     if (lim(lds_lines(mx, b->tz / 2), (b->tz / 2) * mx, kLdsMax) || lim(lds_lines(mz, b->cw / 2), (b->cw / 2) * mz, kLdsMax) ||
         r1_bad(b->tk, kLdsMax))
         return set_err(CQ_EUNSUPPORTED, "server windows up to %dx%d are too large for one workgroup", mx, mz);
-    b->tk2 = lim(lds_lines(mx, 2 * b->tk), 2 * b->tk * mx, kLdsTile) ? b->tk : 2 * b->tk;
+    b->tk2 = b->tk;
+    for (int m : {3, 2})
+        if (m * b->tk <= 16 && !lim(lds_lines(mx, m * b->tk), m * b->tk * mx, kLdsTile)) {
+            b->tk2 = m * b->tk;
+            break;
+        }
     b->maxnxx = mx;
     b->maxnzz = mz;
     set_lds(b, mx, mz, mx, mz, {});
