@@ -302,6 +302,7 @@ struct JobH {
     std::vector<CkptOp> ops;  // RTM checkpoint copies, by local step
     std::vector<int4> hsched; // RTM step records (host copy of d[r].sched)
     size_t op_next = 0;
+    int grp = 0;            // launch group (rebuild_tables)
 };
 
 // One launch of the per-step sequence.
@@ -845,11 +846,12 @@ static int rebuild_tables(cq_batch b)
     b->tmax = 0;
     int nq = 0, na = 0;   // jobs of each kind so far (round-robin over the kind's two groups)
     for (size_t i = 0; i < n; i++) {
-        const JobH &h = b->jobs[i];
+        JobH &h = b->jobs[i];
         hd[i] = h.d[0];
         hd[cap + i] = h.nroles > 1 ? h.d[1] : h.d[0];
         b->tmax = std::max(b->tmax, h.t0 + h.nsched);
         const int g = is_q(h.kind) ? (nq++ & 1) : 2 + (na++ & 1);
+        h.grp = g;
         for (int r = 0; r < h.nroles; r++) {
             const int e = r ? cap + (int)i : (int)i;
             std::vector<int2> *it = items[g][r];
@@ -1025,7 +1027,9 @@ __global__ void k_add_point(float *p, float v) { *p += v; }
 // source (addsource, :1846-1863).
 static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
 {
-    hipStream_t s = b->ctx->stream;
+    // on the stream of the job's launch group: ordered with its own passes,
+    // independent of the other groups
+    hipStream_t s = b->split && h.grp > 0 ? b->side[h.grp] : b->ctx->stream;
     char *ck = h.ckpt + (size_t)op.slot * h.state_bytes;
     if (op.dir) {
         HIPCHK(hipMemcpyAsync(ck, h.state[0], h.state_bytes, hipMemcpyDeviceToDevice, s));
@@ -1079,11 +1083,7 @@ static int advance_steps(cq_batch b, int end)
             const int s = b->t - h.t0;
             while (h.op_next < h.ops.size() && h.ops[h.op_next].at <= s) {
                 if (h.ops[h.op_next].at == s) {
-                    // checkpoint copies run on the context stream, after
-                    // every earlier pass of either chain
-                    int rc = join_side(b);
-                    if (rc) return rc;
-                    rc = ckpt_apply(b, h, h.ops[h.op_next]);
+                    int rc = ckpt_apply(b, h, h.ops[h.op_next]);
                     if (rc) return rc;
                 }
                 h.op_next++;
