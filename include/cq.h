@@ -90,7 +90,9 @@ int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *ou
  * time step per global step until its nt; an RTM shot takes the next step of
  * its own revolve schedule (src.c:314-470: forward source steps, adjoint
  * receiver steps and the checkpoint copies between them), which has
- * numforw + nt steps (cqh_numforw). */
+ * numforw + nt steps (cqh_numforw).  The shots are dealt into four launch
+ * groups that run on the context's stream and three streams of the batch;
+ * they are joined back into the context's stream before this returns. */
 int cq_batch_step(cq_batch b, int nsteps);
 /* Waits for the batch's queued work. */
 int cq_batch_sync(cq_batch b);
