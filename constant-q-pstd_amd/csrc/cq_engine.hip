@@ -730,7 +730,7 @@ static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_
 // better when a power of two does not fit the LDS budget.
 static const int kTzCand[] = {16, 12, 8, 4};
 static const int kCwCand[] = {32, 24, 16, 12, 8, 4, 2};
-static const int kTkCand[] = {8, 4, 2};
+static const int kTkCand[] = {8, 4, 2, 1};
 
 // k_q_r1 holds 3 tk lines and transforms 2 tk at a time (the div and p
 // spectra, then two symbols at a time) when the LDS budget allows it, else
@@ -794,7 +794,7 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
         while (iz < 3 && row_bad(kTzCand[iz], kLdsTile)) iz++;
         while (ick < 6 && col_bad(kCwCand[ick], kLdsTile)) ick++;
         if (q)
-            while (ik < 2 && krow_bad(kTkCand[ik], kLdsTile)) ik++;
+            while (ik < 3 && krow_bad(kTkCand[ik], kLdsTile)) ik++;
         if (row_bad(kTzCand[iz], kLdsMax) || col_bad(kCwCand[ick], kLdsMax) || (q && krow_bad(kTkCand[ik], kLdsMax)))
             return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", (int)i, j.nxx, j.nzz);
         int &mx = q ? qx : ax, &mz = q ? qz : az;
@@ -963,7 +963,9 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
      : T == 12 ? launch(b, K<12>, CQ_ARGS)                                         \
      : T == 8  ? launch(b, K<8>, CQ_ARGS)                                          \
      : T == 6  ? launch(b, K<6>, CQ_ARGS)                                          \
-     : T == 4  ? launch(b, K<4>, CQ_ARGS) : launch(b, K<2>, CQ_ARGS))
+     : T == 4  ? launch(b, K<4>, CQ_ARGS)                                          \
+     : T == 3  ? launch(b, K<3>, CQ_ARGS)                                          \
+     : T == 2  ? launch(b, K<2>, CQ_ARGS) : launch(b, K<1>, CQ_ARGS))
 
 static int launch_pass(cq_batch b, int pass, int role, int step)
 {
@@ -972,8 +974,9 @@ static int launch_pass(cq_batch b, int pass, int role, int step)
     case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, b->cw);
     case P_QC1: return CQ_DISPATCH_CW(k_q_c1, b->cwq);
     case P_QR1:
-        return b->tk == 8 ? launch(b, k_q_r1<8>, CQ_ARGS)
-               : b->tk == 4 ? launch(b, k_q_r1<4>, CQ_ARGS) : launch(b, k_q_r1<2>, CQ_ARGS);
+        return b->tk == 8   ? launch(b, k_q_r1<8>, CQ_ARGS)
+               : b->tk == 4 ? launch(b, k_q_r1<4>, CQ_ARGS)
+               : b->tk == 2 ? launch(b, k_q_r1<2>, CQ_ARGS) : launch(b, k_q_r1<1>, CQ_ARGS);
     case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
     case P_QR2: return CQ_DISPATCH_TK(k_q_r2, b->tk2);
     case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
@@ -1231,7 +1234,7 @@ static int server_tiles(cq_batch b, int mx, int mz)
     };
     while (iz < 3 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
     while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz, kLdsTile)) ic++;
-    while (ik < 2 && r1_bad(kTkCand[ik], kLdsTile)) ik++;
+    while (ik < 3 && r1_bad(kTkCand[ik], kLdsTile)) ik++;
     b->tz = kTzCand[iz];
     b->cw = kCwCand[ic];
     b->cwq = kCwCand[ic];

@@ -137,3 +137,20 @@ def test_very_long_lines_vs_oracle(ctx, kind):
     e = engine.rel_l2(got, orc.forward(job))
     print("rel-L2 gather", e)
     assert e <= TOL
+
+
+def test_single_kz_row_tiles_vs_oracle(ctx):
+    """A 2125-column window: neither two kz rows with their three symbols
+    (6 lines) nor two rows one symbol at a time (4 lines) fit the LDS budget,
+    so the kz-row pass takes one kz row per workgroup (3 lines, div and p
+    transformed together)."""
+    job = synth.small_job(nxt=2104, nzt=24, pml=10, nt=30, kind=1, seed=35)
+    assert job["nxx"] == 2125
+    print("case single kz rows", job["nxx"], job["nzz"], flush=True)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
