@@ -472,7 +472,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.r1_fused = q && r1_fused_for(nxx, tk) ? 1 : 0;
     int rc;
     // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 2 tk
-    // (fused k_q_r1) or tk lines of nxx; column tiles cw/2 lines of nzz
+    // (fused k_q_r1) or tk lines and tk2 lines (k_q_r2) of nxx; column tiles
+    // cw/2 lines of nzz
     const int need_x = std::max((tz / 2) * nxx, q ? std::max((base.r1_fused ? 2 : 1) * tk, tk2) * nxx : 0),
               need_z = (cw / 2) * nzz;
     if ((rc = get_plan(ctx, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
@@ -722,7 +723,7 @@ static const size_t kLdsMax = 160 * 1024;
 static const size_t kLdsTile = CQ_LDS_TILE;
 
 // LDS bytes of a workgroup holding the twiddle table of an n-point transform
-// and `lines` padded lines of it.
+// and `lines` lines of it (cq_line_stride slots apart).
 static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_stride(n)) * 8; }
 // Candidate tile widths, largest first: z rows per row tile (two per packed
 // line), columns per column tile (two per packed line), kz rows per kz-row

@@ -32,7 +32,7 @@ __host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
     int gather;        /* record rec[t][i] (modeling) */                                          \
     float qsign;       /* +1 attenuating (modeling), -1 compensating (Q-RTM) */                   \
     int stab;          /* stabilised |k|^1.01 multiplier for aux (Q-RTM) */                       \
-    int r1_fused;      /* k_q_r1 inverts its three symbols together (else one at a time) */       \
+    int r1_fused;      /* k_q_r1 holds 3 tk lines: symbols inverted two at a time (else one) */   \
     float eta, taoc, dt; /* stabiliser (sgpstd-qrtm/acsglw.cu:1895-1924) */                       \
     float src_scale;   /* dt * 1000 (acps4.cu:1491) */                                            \
     float dkx, dkz, invN; /* wavenumber steps (funca.c:555-556) and 1/(nxx nzz) */                \
