@@ -645,10 +645,16 @@ __device__ __forceinline__ bool cq_step_of(const JobG &J, int step, int &t, int 
     return true;
 }
 
+// The job record is copied by value into registers (J): the kernel's global
+// stores could alias it as far as the compiler knows, so a reference would
+// make it reload the record's fields (scalar loads behind a wait for every
+// outstanding store) in each loop iteration after a store.  The FFT plans,
+// indexed by stage at run time, are read through the reference (Jr).
 #define CQ_ITEM_PROLOGUE                                             \
     const int2 it = items[blockIdx.x];                               \
     if (it.x < 0) return;                                            \
-    const JobG &J = reinterpret_cast<const JobG *>(jobs)[it.x];      \
+    const JobG &Jr = reinterpret_cast<const JobG *>(jobs)[it.x];     \
+    const JobG J = Jr;                                               \
     int t, t_aux, flags;                                             \
     if (!cq_step_of(J, step, t, t_aux, flags)) return;               \
     (void)t_aux;                                                     \
@@ -681,8 +687,8 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD 
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.div, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, LS, L, tw, J.mx2);
-    cq_fft_inv(J.px, buf, LS, L, tw);
+    cq_fft_fwd(Jr.px, buf, LS, L, tw, J.mx2);
+    cq_fft_inv(Jr.px, buf, LS, L, tw);
 
     // p += c1 div - c2 (Lx + Lz); source (addsa_kernel, acps4.cu:1462-1498);
     // illumination ssg += p^2 during the first forward sweep of RTM
@@ -745,12 +751,12 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD 
         }
         __syncthreads();
     }
-    cq_fft_fwd(J.px, buf, LS, L, tw, J.mxp);
-    cq_fft_inv(J.px, buf, LS, L, tw);
+    cq_fft_fwd(Jr.px, buf, LS, L, tw, J.mxp);
+    cq_fft_inv(Jr.px, buf, LS, L, tw);
     row_vx_phase<Q>(J, buf, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, LS, L, tw, J.mxm);
-    cq_fft_inv(J.px, buf, LS, L, tw);
+    cq_fft_fwd(Jr.px, buf, LS, L, tw, J.mxm);
+    cq_fft_inv(Jr.px, buf, LS, L, tw);
     row_dvx_phase<Q>(J, buf, z0);
 }
 
@@ -770,16 +776,16 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_col_ac(const JobD 
     __syncthreads();
     // receiver gather of the updated p (acps4.cu:611-621)
     if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzp);
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzp);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     col_div_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mz2);
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mz2);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
@@ -818,7 +824,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c1(const JobD *_
     load_tw(tw, J.twz, nzz);
     col_fill<CW>(bufs, J, J.div, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -841,30 +847,30 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r1(const JobD *_
         krow_fill<TK>(buf, J, J.s1, k0);
         krow_fill<TK>(buf + TK * LS, J, J.s2, k0);
         __syncthreads();
-        cq_fft_fwd(J.px, buf, LS, 2 * TK, tw, nullptr);
+        cq_fft_fwd(Jr.px, buf, LS, 2 * TK, tw, nullptr);
         krow_round1<TK>(J, buf, k0, aux_symbol, tt);   // aux (lines TK..), K (lines 2 TK..)
         __syncthreads();
-        cq_fft_inv(J.px, buf + TK * LS, LS, 2 * TK, tw);
+        cq_fft_inv(Jr.px, buf + TK * LS, LS, 2 * TK, tw);
         krow_store<TK>(buf + TK * LS, J, J.s2, k0);
         krow_store<TK>(buf + 2 * TK * LS, J, J.s3, k0);
         __syncthreads();
         krow_round2<TK>(J, buf, k0);                   // A (lines 0..), B (lines TK..)
         __syncthreads();
-        cq_fft_inv(J.px, buf, LS, 2 * TK, tw);
+        cq_fft_inv(Jr.px, buf, LS, 2 * TK, tw);
         krow_store<TK>(buf, J, J.s1, k0);
         krow_store<TK>(buf + TK * LS, J, J.s4, k0);
         return;
     }
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
+    cq_fft_fwd(Jr.px, buf, LS, TK, tw, nullptr);
     {
         // one symbol at a time (long lines: 2 TK lines of LDS)
         float2 CQ_GLOBAL *outs[3] = {J.s1, J.s3, J.s4};
         for (int w = 0; w < 3; w++) {
             krow_mult_to<TK>(J, buf, TK, k0, w);
             __syncthreads();
-            cq_fft_inv(J.px, buf + TK * LS, LS, TK, tw);
+            cq_fft_inv(Jr.px, buf + TK * LS, LS, TK, tw);
             krow_store<TK>(buf + TK * LS, J, outs[w], k0);
             __syncthreads();
         }
@@ -872,10 +878,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r1(const JobD *_
     __syncthreads();
     krow_fill<TK>(buf, J, J.s2, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, LS, TK, tw, nullptr);
+    cq_fft_fwd(Jr.px, buf, LS, TK, tw, nullptr);
     krow_mult<TK>(J, buf, k0, aux_symbol, tt);
     __syncthreads();
-    cq_fft_inv(J.px, buf, LS, TK, tw);
+    cq_fft_inv(Jr.px, buf, LS, TK, tw);
     krow_store<TK>(buf, J, J.s2, k0);
 }
 
@@ -911,7 +917,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             if (xh >= J.nxx) break;
             col2_insert<NC>(J, buf, LS, xh, J.s3, J.s4);   // K + i B
             __syncthreads();
-            cq_fft_inv(J.pz, buf, LS, NC, tw);
+            cq_fft_inv(Jr.pz, buf, LS, NC, tw);
             cq_loop<2>(
                 NC * nq,
                 [&](int e) {
@@ -948,7 +954,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             __syncthreads();
             colpair_insert<NP>(J, buf + LS, 2 * LS, xh, J.s1);   // A
             __syncthreads();
-            cq_fft_inv(J.pz, buf + LS, 2 * LS, NP, tw);
+            cq_fft_inv(Jr.pz, buf + LS, 2 * LS, NP, tw);
             cq_loop<4>(
                 NC * nq,
                 [&](int e) {
@@ -979,7 +985,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
                     st4(J.tot + cq_q4(J.nxx, ix, zq), o);
                 });
             __syncthreads();
-            cq_fft_fwd(J.pz, buf + LS, 2 * LS, NP, tw, nullptr);
+            cq_fft_fwd(Jr.pz, buf + LS, 2 * LS, NP, tw, nullptr);
             colpair_extract<NP>(J, buf + LS, 2 * LS, xh, J.s1);
             __syncthreads();
         }
@@ -989,7 +995,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     // `tot` itself; each thread re-reads only the elements it wrote.
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     // column c of the tile: ix = x0 + c; quads beyond nxx are skipped
     auto cell = [&](int e, int &zq, int &c) {
         zq = e / CW;
@@ -1019,7 +1025,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s4);   // B
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     cq_loop<4>(
         CW * nq,
         [&](int e) {
@@ -1049,7 +1055,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s1);   // A
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     cq_loop<3>(
         CW * nq,
         [&](int e) {
@@ -1088,7 +1094,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             st4(J.tot + cq_q4(J.nxx, x0 + c, zq), o);
         });
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -1103,10 +1109,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r2(const JobD *_
     load_tw(tw, J.twx, nxx);
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
+    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
     krow_mult<TK>(J, buf, k0, 3);   // T = |k|^1.01 total (acsglw.cu:578-588)
     __syncthreads();
-    cq_fft_inv(J.px, buf, cq_line_stride(nxx), TK, tw);
+    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), TK, tw);
     krow_store<TK>(buf, J, J.s1, k0);
 }
 
@@ -1123,7 +1129,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     load_tw(tw, J.twz, nzz);
     col_insert<CW>(J, buf, x0, J.s1);   // T
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     // T waits in the scratch field (each thread re-reads only what it wrote)
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
@@ -1140,7 +1146,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     // p += rho v^2 dt lam (total - s dt tao T / 2) - s dt tao aux; source.
     // s = +1 attenuates (acsglw.cu:1500-1537); s = -1 compensates (Q-RTM,
     // sgpstd-qrtm/acsglw.cu:1769-1841).  The new p replaces aux in the
@@ -1210,14 +1216,14 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     }
     if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
     __syncthreads();
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(J.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
-    cq_fft_inv(J.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
+    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
     col_div_phase<CW, true>(J, bufs, x0);
 }
 
@@ -1233,11 +1239,11 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r3(const JobD *_
     load_tw(tw, J.twx, nxx);
     row_fill<Q>(buf, nxx, J.p, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), L, tw, J.mxp);
-    cq_fft_inv(J.px, buf, cq_line_stride(nxx), L, tw);
+    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), L, tw, J.mxp);
+    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), L, tw);
     row_vx_phase<Q>(J, buf, z0);
     __syncthreads();
-    cq_fft_fwd(J.px, buf, cq_line_stride(nxx), L, tw, J.mxm);
-    cq_fft_inv(J.px, buf, cq_line_stride(nxx), L, tw);
+    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), L, tw, J.mxm);
+    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), L, tw);
     row_dvx_phase<Q, true>(J, buf, z0);
 }
