@@ -89,9 +89,11 @@ def test_rtm_linear_in_data(ctx):
 def test_rtm_mixed_schedules_and_modeling_in_one_batch(ctx):
     """Shots with different nt, checkpoint budgets and kinds (RTM, Q-RTM and a
     forward-modeling shot) advance in the same launches, each on its own
-    revolve schedule: every result equals the shot run alone."""
+    revolve schedule: every result equals the shot run alone.  Three RTM
+    shots put two of them in one launch group (one stream, interleaved
+    checkpoint copies)."""
     jobs = [rtm_job(2, nt=60, snaps=4, seed=11), rtm_job(3, nt=90, snaps=7, seed=12),
-            rtm_job(2, nt=75, snaps=75 // 2, seed=13)]
+            rtm_job(2, nt=75, snaps=75 // 2, seed=13), rtm_job(2, nt=50, snaps=3, seed=15)]
     fwd = synth.small_job(nxt=40, nzt=24, pml=5, nt=50, kind=1, seed=14)
     alone = []
     for j in jobs:
@@ -127,3 +129,24 @@ def test_qrtm_long_lines_vs_oracle(ctx):
     ei, es = engine.rel_l2(img, ref_img), engine.rel_l2(ssg, ref_ssg)
     print("rel-L2 image", ei, "illumination", es)
     assert es <= TOL_SSG and ei <= TOL_IMG
+
+
+def test_mixed_batch_in_uneven_chunks_equals_run(ctx):
+    """The launch groups' streams fork and join at every cq_batch_step call:
+    advancing a mixed batch in uneven chunks gives the results of one run."""
+    jobs = [rtm_job(2, nt=40, snaps=3, seed=21), rtm_job(3, nt=50, snaps=5, seed=22),
+            synth.small_job(nxt=40, nzt=24, pml=5, nt=45, kind=0, seed=23),
+            synth.small_job(nxt=40, nzt=24, pml=5, nt=35, kind=1, seed=24)]
+    b = ctx.batch(jobs)
+    b.run()
+    ref = [b.image(0), b.image(1), b.rec(2), b.rec(3)]
+    b.close()
+    b = ctx.batch(jobs)
+    for k in [1, 7, 3, 13, 2, 29, 5] * 20:
+        b.step(k)
+    b.sync()
+    got = [b.image(0), b.image(1), b.rec(2), b.rec(3)]
+    b.close()
+    for i in range(2):
+        assert np.array_equal(got[i][0], ref[i][0]) and np.array_equal(got[i][1], ref[i][1]), i
+    assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
