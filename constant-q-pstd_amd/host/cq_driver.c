@@ -116,21 +116,6 @@ int main(int argc, char **argv)
     for (int s = 0; s < shotnum; s++)
         if (shotxp[s] < 0 || shotxp[s] > xmax || shotzp[s] < 0 || shotzp[s] > zmax)
             DIE("%d shot exceeds model!", s);
-    int nx = 0;
-    const float sroffmx = cqh_window_sroffmx(0, shotnum, tracenum, shotxp, P.fname4, &nx, P.disx, dx);
-    if (sroffmx < 0) DIE("Cannot open receiver positions file %s<shot>.bin", P.fname4);
-    const int nz = nzt, nxx = nx + 2 * pml, nzz = nz + 2 * pml;
-    printf(" window %d x %d (padded %d x %d), %d shots, tmax %d\n", nx, nz, nxx, nzz, shotnum, tmax);
-
-    float *iax = (float *)malloc(4 * nxx), *ibx = (float *)malloc(4 * nxx), *hax = (float *)malloc(4 * nxx),
-          *hbx = (float *)malloc(4 * nxx), *iaz = (float *)malloc(4 * nzz), *ibz = (float *)malloc(4 * nzz),
-          *haz = (float *)malloc(4 * nzz), *hbz = (float *)malloc(4 * nzz);
-    cqh_pml_coeffs(pml, fd, iax, ibx, iaz, ibz, hax, hbx, haz, hbz, nxx, nzz, dx, dz, dt);
-    float *kxsmr = (float *)malloc(4 * nxx), *kxscr = (float *)malloc(4 * nxx), *kxsmi = (float *)malloc(4 * nxx),
-          *kxsci = (float *)malloc(4 * nxx), *kzsmr = (float *)malloc(4 * nzz), *kzscr = (float *)malloc(4 * nzz),
-          *kzsmi = (float *)malloc(4 * nzz), *kzsci = (float *)malloc(4 * nzz);
-    cqh_shift_coeffs(kxsmr, kxscr, kzsmr, kzscr, kxsmi, kxsci, kzsmi, kzsci, dx, dz, nxx, nzz);
-
     cq_ctx ctx;
     if (cq_open(0, &ctx)) DIE("cq_open: %s", cq_last_error());
     int nrmx = 0;
@@ -141,121 +126,159 @@ int main(int argc, char **argv)
         img_sp = (float *)calloc(nzxt, sizeof(float));
         ssg_sp = (float *)calloc(nzxt, sizeof(float));
     }
+    /* The reference splits the shots over numprocs MPI ranks of GPU_N GPUs each
+     * (acousticsgps4.c:255-273): rank r runs shots offsp .. offsp + nshot GPU_N - 1
+     * with a window sized over its own shots, and the last shotnum % (numprocs
+     * GPU_N) % GPU_N shots run nowhere.  CQ_SHOT_SPLIT="numprocs,GPU_N" reproduces
+     * that split (window per rank, dropped shots); by default every shot runs
+     * with one window over all shots. */
+    int np = 1, gpn = 1, split = 0;
+    const char *sp = getenv("CQ_SHOT_SPLIT");
+    if (sp && sscanf(sp, "%d,%d", &np, &gpn) == 2 && np > 0 && gpn > 0) split = 1;
+    else np = gpn = 1;
     const double t0 = now_s();
-    const int chunk = rtm ? shotnum : (batch > 0 ? batch : 64);
-    for (int s0 = 0; s0 < shotnum; s0 += chunk) {
-        const int ns = (s0 + chunk <= shotnum) ? chunk : shotnum - s0;
-        shot_buf *sb = (shot_buf *)calloc(ns, sizeof(shot_buf));
-        cq_job_desc *jd = (cq_job_desc *)calloc(ns, sizeof(cq_job_desc));
-        for (int k = 0; k < ns; k++) {
-            const int sid = s0 + k;
-            shot_buf *b = &sb[k];
-            b->shotid = sid;
-            b->nr = tracenum[sid];
-            const float xs = shotxp[sid], zs = shotzp[sid];
-            const float mlx = xs - sroffmx - P.disx;
-            b->vp = win_field(vpt, nxt, nzt, nx, nz, pml, mlx, dx);
-            b->rho = win_field(rhot, nxt, nzt, nx, nz, pml, mlx, dx);
-            b->bbx = win_field(bbxt, nxt, nzt, nx, nz, pml, mlx, dx);
-            b->bbz = win_field(bbzt, nxt, nzt, nx, nz, pml, mlx, dx);
-            if (q) {
-                b->gam = win_field(gamt, nxt, nzt, nx, nz, pml, mlx, dx);
-                b->lam = win_field(lamt, nxt, nzt, nx, nz, pml, mlx, dx);
-                b->vwd = win_field(vwdt, nxt, nzt, nx, nz, pml, mlx, dx);
-                b->tao = win_field(taot, nxt, nzt, nx, nz, pml, mlx, dx);
-            }
-            if (cqh_read_geophones(P.fname4, sid, b->nr, gxp) || cqh_read_geophones(P.fname5, sid, b->nr, gzp))
-                DIE("Cannot open receiver positions file for shot %d", sid);
-            b->gxg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
-            b->gzg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
-            if (cqh_shot_indices(pml, nxx, nzz, dx, dz, sroffmx, P.disx, xs, zs, b->nr, gxp, gzp, b->gxg, b->gzg,
-                                 &b->geo))
-                DIE("%d shot grid index exceeds model!", sid);
-            if (b->geo.nclamped) printf("warning: %d receivers of shot %d clamped out of the x pml\n", b->geo.nclamped, sid);
-            if (rtm) {
-                char name[512];
-                b->obs = (float *)malloc(sizeof(float) * (size_t)tmax * b->nr);
-                snprintf(name, sizeof name, "%s%d_%d.bin", P.recname, sid, b->nr);
-                if (cqh_read_f32(name, b->obs, (long)tmax * b->nr)) DIE("Cannot open observed data %s", name);
-                cqh_removedirect(tmax, b->nr, dt, fd, xs, zs, gxp, gzp, P.v0, b->obs);
-            }
-            cq_job_desc *j = &jd[k];
-            j->kind = rtm ? (q ? CQ_KIND_QRTM : CQ_KIND_RTM) : (q ? CQ_KIND_VISCO : CQ_KIND_ACOUSTIC);
-            j->nxx = nxx; j->nzz = nzz; j->pml = pml; j->nt = tmax; j->dt = dt;
-            j->vp = b->vp; j->rho = b->rho; j->bbx = b->bbx; j->bbz = b->bbz;
-            j->gam = b->gam; j->lam = b->lam; j->vwd = b->vwd; j->tao = b->tao;
-            j->iax = iax; j->ibx = ibx; j->hax = hax; j->hbx = hbx;
-            j->iaz = iaz; j->ibz = ibz; j->haz = haz; j->hbz = hbz;
-            j->kxsmr = kxsmr; j->kxsmi = kxsmi; j->kxscr = kxscr; j->kxsci = kxsci;
-            j->kzsmr = kzsmr; j->kzsmi = kzsmi; j->kzscr = kzscr; j->kzsci = kzsci;
-            j->dx = dx; j->dz = dz; j->rik = rik;
-            j->xsg = b->geo.xsg; j->zsg = b->geo.zsg;
-            j->nr = b->nr; j->gxg = b->gxg; j->gzg = b->gzg;
-            j->obs = b->obs; j->snaps = P.snapnum; j->etas = P.etas; j->etar = P.etar;
-            printf("shotid=%d nxx=%d nzz=%d xs=%f zs=%f\n", sid, nxx, nzz, xs, zs);
+    for (int rank = 0; rank < np; rank++) {
+        int g0 = 0, g1 = shotnum;
+        if (split) {
+            const int rid = shotnum % (np * gpn), ave = (shotnum - rid) / (np * gpn), ridg = rid / gpn;
+            const int nshot = rank < ridg ? ave + 1 : ave;
+            g0 = rank < ridg ? rank * (ave + 1) * gpn : ridg * (ave + 1) * gpn + (rank - ridg) * ave * gpn;
+            g1 = g0 + nshot * gpn;
+            if (rank == np - 1) printf("Run total shot number is %d\n", g1);
+            if (g1 <= g0) continue;
         }
-        cq_batch bt;
-        if (cq_batch_create(ctx, jd, ns, &bt)) DIE("cq_batch_create: %s", cq_last_error());
-        if (!rtm && P.flagout == 1) {
-            /* snapshots (sgpstd-qmodeling/acps4.cu:482-494, 624-640): the
-             * interior of p after every step t with t % 10 == 0, appended
-             * x-major to ../obsdata/snapa<shot>_<nz>_<nx>.bin (snapva for the
-             * viscoacoustic binary), each file truncated first */
-            float *pf = (float *)malloc(sizeof(float) * (size_t)nxx * nzz);
-            char name[512];
+        int nx = 0;
+        const float sroffmx = cqh_window_sroffmx(g0, g1, tracenum, shotxp, P.fname4, &nx, P.disx, dx);
+        if (sroffmx < 0) DIE("Cannot open receiver positions file %s<shot>.bin", P.fname4);
+        const int nz = nzt, nxx = nx + 2 * pml, nzz = nz + 2 * pml;
+        printf(" window %d x %d (padded %d x %d), %d shots, tmax %d\n", nx, nz, nxx, nzz, g1 - g0, tmax);
+
+        float *iax = (float *)malloc(4 * nxx), *ibx = (float *)malloc(4 * nxx), *hax = (float *)malloc(4 * nxx),
+              *hbx = (float *)malloc(4 * nxx), *iaz = (float *)malloc(4 * nzz), *ibz = (float *)malloc(4 * nzz),
+              *haz = (float *)malloc(4 * nzz), *hbz = (float *)malloc(4 * nzz);
+        cqh_pml_coeffs(pml, fd, iax, ibx, iaz, ibz, hax, hbx, haz, hbz, nxx, nzz, dx, dz, dt);
+        float *kxsmr = (float *)malloc(4 * nxx), *kxscr = (float *)malloc(4 * nxx), *kxsmi = (float *)malloc(4 * nxx),
+              *kxsci = (float *)malloc(4 * nxx), *kzsmr = (float *)malloc(4 * nzz), *kzscr = (float *)malloc(4 * nzz),
+              *kzsmi = (float *)malloc(4 * nzz), *kzsci = (float *)malloc(4 * nzz);
+        cqh_shift_coeffs(kxsmr, kxscr, kzsmr, kzscr, kxsmi, kxsci, kzsmi, kzsci, dx, dz, nxx, nzz);
+
+        const int chunk = rtm ? g1 - g0 : (batch > 0 ? batch : 64);
+        for (int s0 = g0; s0 < g1; s0 += chunk) {
+            const int ns = (s0 + chunk <= g1) ? chunk : g1 - s0;
+            shot_buf *sb = (shot_buf *)calloc(ns, sizeof(shot_buf));
+            cq_job_desc *jd = (cq_job_desc *)calloc(ns, sizeof(cq_job_desc));
             for (int k = 0; k < ns; k++) {
-                snprintf(name, sizeof name, "%s/%s%d_%d_%d.bin", outdir, q ? "snapva" : "snapa", sb[k].shotid, nz, nx);
-                FILE *fs = fopen(name, "wb");
-                if (!fs) DIE("cannot write %s", name);
-                fclose(fs);
+                const int sid = s0 + k;
+                shot_buf *b = &sb[k];
+                b->shotid = sid;
+                b->nr = tracenum[sid];
+                const float xs = shotxp[sid], zs = shotzp[sid];
+                const float mlx = xs - sroffmx - P.disx;
+                b->vp = win_field(vpt, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->rho = win_field(rhot, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->bbx = win_field(bbxt, nxt, nzt, nx, nz, pml, mlx, dx);
+                b->bbz = win_field(bbzt, nxt, nzt, nx, nz, pml, mlx, dx);
+                if (q) {
+                    b->gam = win_field(gamt, nxt, nzt, nx, nz, pml, mlx, dx);
+                    b->lam = win_field(lamt, nxt, nzt, nx, nz, pml, mlx, dx);
+                    b->vwd = win_field(vwdt, nxt, nzt, nx, nz, pml, mlx, dx);
+                    b->tao = win_field(taot, nxt, nzt, nx, nz, pml, mlx, dx);
+                }
+                if (cqh_read_geophones(P.fname4, sid, b->nr, gxp) || cqh_read_geophones(P.fname5, sid, b->nr, gzp))
+                    DIE("Cannot open receiver positions file for shot %d", sid);
+                b->gxg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
+                b->gzg = (int *)malloc(sizeof(int) * (b->nr > 0 ? b->nr : 1));
+                if (cqh_shot_indices(pml, nxx, nzz, dx, dz, sroffmx, P.disx, xs, zs, b->nr, gxp, gzp, b->gxg, b->gzg,
+                                     &b->geo))
+                    DIE("%d shot grid index exceeds model!", sid);
+                if (b->geo.nclamped) printf("warning: %d receivers of shot %d clamped out of the x pml\n", b->geo.nclamped, sid);
+                if (rtm) {
+                    char name[512];
+                    b->obs = (float *)malloc(sizeof(float) * (size_t)tmax * b->nr);
+                    snprintf(name, sizeof name, "%s%d_%d.bin", P.recname, sid, b->nr);
+                    if (cqh_read_f32(name, b->obs, (long)tmax * b->nr)) DIE("Cannot open observed data %s", name);
+                    cqh_removedirect(tmax, b->nr, dt, fd, xs, zs, gxp, gzp, P.v0, b->obs);
+                }
+                cq_job_desc *j = &jd[k];
+                j->kind = rtm ? (q ? CQ_KIND_QRTM : CQ_KIND_RTM) : (q ? CQ_KIND_VISCO : CQ_KIND_ACOUSTIC);
+                j->nxx = nxx; j->nzz = nzz; j->pml = pml; j->nt = tmax; j->dt = dt;
+                j->vp = b->vp; j->rho = b->rho; j->bbx = b->bbx; j->bbz = b->bbz;
+                j->gam = b->gam; j->lam = b->lam; j->vwd = b->vwd; j->tao = b->tao;
+                j->iax = iax; j->ibx = ibx; j->hax = hax; j->hbx = hbx;
+                j->iaz = iaz; j->ibz = ibz; j->haz = haz; j->hbz = hbz;
+                j->kxsmr = kxsmr; j->kxsmi = kxsmi; j->kxscr = kxscr; j->kxsci = kxsci;
+                j->kzsmr = kzsmr; j->kzsmi = kzsmi; j->kzscr = kzscr; j->kzsci = kzsci;
+                j->dx = dx; j->dz = dz; j->rik = rik;
+                j->xsg = b->geo.xsg; j->zsg = b->geo.zsg;
+                j->nr = b->nr; j->gxg = b->gxg; j->gzg = b->gzg;
+                j->obs = b->obs; j->snaps = P.snapnum; j->etas = P.etas; j->etar = P.etar;
+                printf("shotid=%d nxx=%d nzz=%d xs=%f zs=%f\n", sid, nxx, nzz, xs, zs);
             }
-            for (int t = 0; t < tmax; t++) {
-                if (cq_batch_step(bt, 1)) DIE("cq_batch_step: %s", cq_last_error());
-                if (t % 10) continue;
+            cq_batch bt;
+            if (cq_batch_create(ctx, jd, ns, &bt)) DIE("cq_batch_create: %s", cq_last_error());
+            if (!rtm && P.flagout == 1) {
+                /* snapshots (sgpstd-qmodeling/acps4.cu:482-494, 624-640): the
+                 * interior of p after every step t with t % 10 == 0, appended
+                 * x-major to ../obsdata/snapa<shot>_<nz>_<nx>.bin (snapva for the
+                 * viscoacoustic binary), each file truncated first */
+                float *pf = (float *)malloc(sizeof(float) * (size_t)nxx * nzz);
+                char name[512];
                 for (int k = 0; k < ns; k++) {
-                    if (cq_batch_read_p(bt, k, pf)) DIE("cq_batch_read_p: %s", cq_last_error());
                     snprintf(name, sizeof name, "%s/%s%d_%d_%d.bin", outdir, q ? "snapva" : "snapa", sb[k].shotid, nz, nx);
-                    FILE *fs = fopen(name, "ab");
+                    FILE *fs = fopen(name, "wb");
                     if (!fs) DIE("cannot write %s", name);
-                    for (int ix = pml; ix < nxx - pml; ix++)
-                        if (fwrite(pf + (size_t)ix * nzz + pml, sizeof(float), (size_t)nz, fs) != (size_t)nz)
-                            DIE("short write of %s", name);
                     fclose(fs);
                 }
+                for (int t = 0; t < tmax; t++) {
+                    if (cq_batch_step(bt, 1)) DIE("cq_batch_step: %s", cq_last_error());
+                    if (t % 10) continue;
+                    for (int k = 0; k < ns; k++) {
+                        if (cq_batch_read_p(bt, k, pf)) DIE("cq_batch_read_p: %s", cq_last_error());
+                        snprintf(name, sizeof name, "%s/%s%d_%d_%d.bin", outdir, q ? "snapva" : "snapa", sb[k].shotid, nz, nx);
+                        FILE *fs = fopen(name, "ab");
+                        if (!fs) DIE("cannot write %s", name);
+                        for (int ix = pml; ix < nxx - pml; ix++)
+                            if (fwrite(pf + (size_t)ix * nzz + pml, sizeof(float), (size_t)nz, fs) != (size_t)nz)
+                                DIE("short write of %s", name);
+                        fclose(fs);
+                    }
+                }
+                free(pf);
             }
-            free(pf);
-        }
-        if (cq_batch_run(bt)) DIE("cq_batch_run: %s", cq_last_error());
-        for (int k = 0; k < ns; k++) {
-            shot_buf *b = &sb[k];
-            if (!rtm) {
-                char name[512];
-                float *rec = (float *)malloc(sizeof(float) * (size_t)tmax * (b->nr > 0 ? b->nr : 1));
-                if (cq_batch_read_rec(bt, k, rec)) DIE("cq_batch_read_rec: %s", cq_last_error());
-                snprintf(name, sizeof name, "%s/%s%d_%d.bin", outdir, q ? "recva" : "reca", b->shotid, b->nr);
-                if (cqh_write_f32(name, rec, (long)tmax * b->nr)) DIE("cannot write %s", name);
-                free(rec);
-            } else {
-                /* post-processing of one shot (sgpstd-qrtm/acps4.cu:909-945) */
-                float *img = (float *)malloc(sizeof(float) * (size_t)nx * nz);
-                float *imgf = (float *)malloc(sizeof(float) * (size_t)nx * nz);
-                float *ssg = (float *)malloc(sizeof(float) * (size_t)nx * nz);
-                float *pj = (float *)calloc(nzxt, sizeof(float));
-                if (cq_batch_read_image(bt, k, img, ssg)) DIE("cq_batch_read_image: %s", cq_last_error());
-                cqh_filter_image(nx, nz, dx, dz, img, imgf);
-                cqh_mute_image(nx, nz, pml, b->geo.lgest, b->geo.rgest, imgf);
-                cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, imgf);
-                for (size_t i = 0; i < nzxt; i++) img_sp[i] += pj[i];
-                memset(pj, 0, sizeof(float) * nzxt);
-                cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, ssg);
-                for (size_t i = 0; i < nzxt; i++) ssg_sp[i] += pj[i];
-                free(img); free(imgf); free(ssg); free(pj);
+            if (cq_batch_run(bt)) DIE("cq_batch_run: %s", cq_last_error());
+            for (int k = 0; k < ns; k++) {
+                shot_buf *b = &sb[k];
+                if (!rtm) {
+                    char name[512];
+                    float *rec = (float *)malloc(sizeof(float) * (size_t)tmax * (b->nr > 0 ? b->nr : 1));
+                    if (cq_batch_read_rec(bt, k, rec)) DIE("cq_batch_read_rec: %s", cq_last_error());
+                    snprintf(name, sizeof name, "%s/%s%d_%d.bin", outdir, q ? "recva" : "reca", b->shotid, b->nr);
+                    if (cqh_write_f32(name, rec, (long)tmax * b->nr)) DIE("cannot write %s", name);
+                    free(rec);
+                } else {
+                    /* post-processing of one shot (sgpstd-qrtm/acps4.cu:909-945) */
+                    float *img = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                    float *imgf = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                    float *ssg = (float *)malloc(sizeof(float) * (size_t)nx * nz);
+                    float *pj = (float *)calloc(nzxt, sizeof(float));
+                    if (cq_batch_read_image(bt, k, img, ssg)) DIE("cq_batch_read_image: %s", cq_last_error());
+                    cqh_filter_image(nx, nz, dx, dz, img, imgf);
+                    cqh_mute_image(nx, nz, pml, b->geo.lgest, b->geo.rgest, imgf);
+                    cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, imgf);
+                    for (size_t i = 0; i < nzxt; i++) img_sp[i] += pj[i];
+                    memset(pj, 0, sizeof(float) * nzxt);
+                    cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, ssg);
+                    for (size_t i = 0; i < nzxt; i++) ssg_sp[i] += pj[i];
+                    free(img); free(imgf); free(ssg); free(pj);
+                }
+                free_shot(b);
             }
-            free_shot(b);
+            cq_batch_destroy(bt);
+            free(sb);
+            free(jd);
         }
-        cq_batch_destroy(bt);
-        free(sb);
-        free(jd);
+        free(iax); free(ibx); free(hax); free(hbx); free(iaz); free(ibz); free(haz); free(hbz);
+        free(kxsmr); free(kxscr); free(kxsmi); free(kxsci); free(kzsmr); free(kzscr); free(kzsmi); free(kzsci);
     }
     const double el = now_s() - t0;
     printf("The cost of the run time is %f seconds\n", el);

@@ -117,3 +117,29 @@ def test_rtm_driver_writes_stack(tree, mode, out):
     img, ssg = img[:NXT * NZT], img[NXT * NZT:]
     assert np.isfinite(img).all() and np.isfinite(ssg).all()
     assert np.abs(img).max() > 0 and ssg.min() >= 0 and ssg.max() > 0
+
+
+@pytest.mark.parametrize("split,total", [("1,2", 2), ("2,1", 3)])
+def test_reference_shot_split(tree, split, total):
+    """CQ_SHOT_SPLIT="numprocs,GPU_N" runs the shots the reference's MPI split
+    would run (acousticsgps4.c:255-273): 3 shots on 1 rank x 2 GPUs run shots
+    0 and 1 only (the remainder shot is dropped), on 2 ranks x 1 GPU all three.
+    Every shot that runs writes the gather of the default run."""
+    t, vp, rho, q = tree
+    out = t / ("obsdata_split_" + split.replace(",", "_"))
+    out.mkdir(exist_ok=True)
+    print("driver aps split", split, flush=True)
+    r = subprocess.run([os.path.join(BIN, "aps"), "../input/aps.txt", str(out), "2"], cwd=t / "work",
+                       capture_output=True, text=True, timeout=240, env=dict(os.environ, CQ_SHOT_SPLIT=split))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Run total shot number is %d" % total in r.stdout
+    nr = 2 * HALF + 1
+    for i in range(len(SHOTS)):
+        f = out / ("reca%d_%d.bin" % (i, nr))
+        if i >= total:
+            assert not f.exists()
+            continue
+        got = np.fromfile(f, np.float32).reshape(NT, nr)
+        e = engine.rel_l2(got, orc.forward(shot_job(0, vp, rho, q, SHOTS[i])))
+        print("split", split, "shot", i, "rel-L2 vs FP64 oracle", e)
+        assert e <= 1e-4
