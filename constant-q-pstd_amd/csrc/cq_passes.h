@@ -63,7 +63,7 @@ struct F4x5 {
 
 __device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__restrict__ g, int n)
 {
-    for (int i = threadIdx.x; i < n; i += CQ_NT) tw[i] = g[i];
+    cq_loop<4>(n, [&](int i) { return g[i]; }, [&](int i, float2 v) { tw[i] = v; });
 }
 
 // ===========================================================================
