@@ -30,6 +30,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -926,12 +927,26 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     return CQ_OK;
 }
 
+// The dynamic-LDS limit is a per-kernel, per-device attribute shared by every
+// context of the process: raise it when a launch needs more (one host call
+// per kernel and size instead of one per launch); it never shrinks.
+static hipError_t raise_lds_limit(int device, const void *kernel, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> limit;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &cur = limit[std::make_pair(device, kernel)];
+    if (lds <= cur) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) cur = lds;
+    return e;
+}
+
 template <typename K>
 static int launch(cq_batch b, K kernel, int pass, int role, int step)
 {
     const size_t lds = b->lds[pass];
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIPCHK(raise_lds_limit(b->ctx->device, reinterpret_cast<const void *>(kernel), lds));
     // one launch per group on its stream, or, unsplit, one for all groups
     for (int g = 0; g < kGroups; g++) {
         const int o = b->split ? b->goff[role][pass][g] : 0;
