@@ -8,9 +8,11 @@ viscoacoustic.  One "step" advances every shot of the batch by one time step
 (2 fused passes per acoustic shot, 6 per viscoacoustic shot).  All inputs are
 resident in HBM before the timed region.
 
-Reported: batched Mcell-steps/s (the metric), the plain one-shot-per-launch
-path on one shot of the same workload, the dominant pass's HBM roofline
-fraction (HIP-event timing on the engine's stream) with its measured traffic,
+Reported: batched Mcell-steps/s (the metric; the timed steps run the batch's
+four launch groups on four streams), the plain one-shot-per-launch path on
+one shot of the same workload, the dominant pass's HBM roofline fraction
+(HIP-event timing on the engine's stream over extra steps that launch each
+pass once over every shot) with its measured traffic,
 the FP32 reference-faithful CPU path (the oracle's FP32 build: 9 full 2-D C2C
 FFTs per step, OpenMP) timed on a bounded sample of the same workload, the
 serving phase (continuous batching of Poisson arrivals: throughput, p50/p99
