@@ -154,3 +154,20 @@ def test_single_kz_row_tiles_vs_oracle(ctx):
     e = engine.rel_l2(got, orc.forward(job))
     print("rel-L2 gather", e)
     assert e <= TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_tall_window_vs_oracle(ctx, kind):
+    """A 640-depth window: column tiles of 24 columns (12 packed lines; the
+    viscoacoustic column pass then runs its partial sums in LDS, two halves
+    of 12 columns), and an odd width whose last tile is partial."""
+    job = synth.small_job(nxt=44, nzt=620, pml=10, nt=40, kind=kind, seed=37)
+    print("case tall window kind", kind, job["nxx"], job["nzz"], flush=True)
+    assert job["nzz"] == 640 and job["nxx"] == 65
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
