@@ -171,3 +171,21 @@ def test_tall_window_vs_oracle(ctx, kind):
     e = engine.rel_l2(got, orc.forward(job))
     print("rel-L2 gather", e)
     assert e <= TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_benchmark_window_vs_oracle(ctx, kind):
+    """One interior shot of the batched benchmark workload at its full
+    1296 x 680 padded window (12-depth row tiles, 24-column tiles, two kz
+    rows per kz-row workgroup), 150 steps, against the FP64 oracle."""
+    job = synth.batch128_jobs(nt=150, nshots=128, kinds=[kind] * 128, want_k2=True, shots=[64])[0]
+    print("case benchmark window kind", kind, job["nxx"], job["nzz"], flush=True)
+    assert (job["nxx"], job["nzz"]) == (1296, 680)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    orc.set_threads(16)
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
