@@ -20,7 +20,9 @@
 //     there), so they are stored for the strips only.
 //   * Shots of different sizes and kinds advance in the same launches: the
 //     grid is a list of (shot, tile) work items over a device table of shots,
-//     ordered so neighbouring tiles run on the same XCD.
+//     ordered so neighbouring tiles run on the same XCD.  The shots are dealt
+//     into four launch groups (two per kind) with their own streams, so the
+//     kernels of different groups overlap on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
