@@ -12,8 +12,28 @@
 #pragma once
 #include "cq_job.h"
 
-__device__ __forceinline__ float4 ld4(const float CQ_GLOBAL *p) { return *reinterpret_cast<const float4 CQ_GLOBAL *>(p); }
-__device__ __forceinline__ void st4(float CQ_GLOBAL *p, float4 v) { *reinterpret_cast<float4 CQ_GLOBAL *>(p) = v; }
+// Field and spectrum traffic is streamed: every pass touches each cell once
+// and the batch's working set is far beyond L2 and the Infinity Cache, so the
+// quads are loaded and stored with the nontemporal hint (2-4% off the column
+// and update passes).  Tables that every workgroup re-reads (twiddles,
+// multipliers, CPML profiles) keep ordinary loads.
+typedef float cq_v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float CQ_GLOBAL *p)
+{
+    const cq_v4 w = __builtin_nontemporal_load(reinterpret_cast<const cq_v4 CQ_GLOBAL *>(p));
+    return make_float4(w.x, w.y, w.z, w.w);
+}
+__device__ __forceinline__ void st4(float CQ_GLOBAL *p, float4 v)
+{
+    const cq_v4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<cq_v4 CQ_GLOBAL *>(p));
+}
+// ordinary (cached) accesses, for quads a workgroup reads twice or writes and
+// reads back within one pass, so that the second access can hit in L2
+__device__ __forceinline__ float4 ld4c(const float CQ_GLOBAL *p) { return *reinterpret_cast<const float4 CQ_GLOBAL *>(p); }
+__device__ __forceinline__ void st4c(float CQ_GLOBAL *p, float4 v) { *reinterpret_cast<float4 CQ_GLOBAL *>(p) = v; }
+// two complex half-spectrum values
+__device__ __forceinline__ void st4v(float2 CQ_GLOBAL *p, float4 v) { st4(reinterpret_cast<float CQ_GLOBAL *>(p), v); }
 __device__ __forceinline__ float f4get(const float4 &v, int k)
 {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
@@ -353,7 +373,7 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
             // columns ix0, ix0 + 1 share a block (ix0 even); with odd nxx the
             // last pair's second column is a block's spare column
             if (ix0 < J.nxx)
-                *reinterpret_cast<float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k)) = make_float4(g0.x, g0.y, g1.x, g1.y);
+                st4v(S + cq_hs(ldk, ix0, k), make_float4(g0.x, g0.y, g1.x, g1.y));
             if (mh != nullptr) {
                 line[k] = cmul(zk, make_float2(m.x, m.y));
                 if (k2 != k) line[k2] = cmul(zm, make_float2(m.z, m.w));
@@ -470,8 +490,7 @@ __device__ __forceinline__ void colpair_extract(const JobG &J, const float2 *buf
         const float2 zk = line[k], zm = line[k2];
         const int ix0 = x0 + 2 * l;
         if (ix0 < J.nxx)
-            *reinterpret_cast<float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k)) =
-                make_float4(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y), 0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+            st4v(S + cq_hs(ldk, ix0, k), make_float4(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y), 0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x)));
     }
 }
 
@@ -514,8 +533,7 @@ __device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, flo
         const float2 *line = buf + (w >> 2) * LS;
         const float2 a = ix < nxx ? line[ix] : make_float2(0.f, 0.f);
         const float2 c = ix + 1 < nxx ? line[ix + 1] : make_float2(0.f, 0.f);
-        *reinterpret_cast<float4 CQ_GLOBAL *>(S + ((size_t)b * J.ldk + k0 + (w >> 2)) * 8 + 2 * (w & 3)) =
-            make_float4(a.x, a.y, c.x, c.y);
+        st4v(S + ((size_t)b * J.ldk + k0 + (w >> 2)) * 8 + 2 * (w & 3), make_float4(a.x, a.y, c.x, c.y));
     }
 }
 
@@ -925,10 +943,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
                     F4x5 r;
                     if (ix < J.nxx) {
                         const size_t g = cq_q4(J.nxx, ix, zq);
-                        r.a = ld4(J.qg + g);
-                        r.b = ld4(J.qy2 + g);
-                        r.c = ld4(J.qcc + g);
-                        r.d = ld4(J.div + g);
+                        r.a = ld4c(J.qg + g);
+                        r.b = ld4c(J.qy2 + g);
+                        r.c = ld4c(J.qcc + g);
+                        r.d = ld4c(J.div + g);
                     }
                     return r;
                 },
@@ -960,7 +978,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
                 [&](int e) {
                     const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
                     float4 y2 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (ix < J.nxx) y2 = ld4(J.qy2 + cq_q4(J.nxx, ix, zq));
+                    if (ix < J.nxx) y2 = ld4c(J.qy2 + cq_q4(J.nxx, ix, zq));
                     return y2;
                 },
                 [&](int e, float4 y2) {
@@ -1007,7 +1025,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         [&](int e) {
             int zq, c;
             float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (cell(e, zq, c)) gm = ld4(J.qg + cq_q4(J.nxx, x0 + c, zq));
+            if (cell(e, zq, c)) gm = ld4c(J.qg + cq_q4(J.nxx, x0 + c, zq));
             return gm;
         },
         [&](int e, float4 gm) {
@@ -1033,9 +1051,9 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             F4x3 r;
             if (cell(e, zq, c)) {
                 const size_t g = cq_q4(J.nxx, x0 + c, zq);
-                r.a = ld4(J.qy2 + g);
-                r.b = ld4(J.qcc + g);
-                r.c = ld4(J.tot + g);
+                r.a = ld4c(J.qy2 + g);
+                r.b = ld4c(J.qcc + g);
+                r.c = ld4c(J.tot + g);
             }
             return r;
         },
@@ -1063,10 +1081,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             F4x5 r;
             if (cell(e, zq, c)) {
                 const size_t g = cq_q4(J.nxx, x0 + c, zq);
-                r.a = ld4(J.qg + g);
-                r.b = ld4(J.qy2 + g);
-                r.c = ld4(J.div + g);
-                r.d = ld4(J.tot + g);
+                r.a = ld4c(J.qg + g);
+                r.b = ld4c(J.qy2 + g);
+                r.c = ld4c(J.div + g);
+                r.d = ld4c(J.tot + g);
             }
             return r;
         },
@@ -1141,7 +1159,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
             const int iz = 4 * zq + k;
             if (iz < nzz) f4set(a, k, line[2 * iz]);
         }
-        st4(J.scr + cq_q4(J.nxx, ix, zq), a);
+        st4c(J.scr + cq_q4(J.nxx, ix, zq), a);
     }
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
@@ -1169,7 +1187,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
                 r.qa = ld4(J.qa + g);
                 r.qb = ld4(J.qb + g);
                 r.qe = ld4(J.qe + g);
-                r.tt = ld4(J.scr + g);
+                r.tt = ld4c(J.scr + g);
             }
             return r;
         },
