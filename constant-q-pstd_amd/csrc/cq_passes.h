@@ -89,13 +89,18 @@ __device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__re
 // ===========================================================================
 // row helpers (real rows, packed)
 // ===========================================================================
-template <int Q>
+// CACHED: the pass reads src again later (ordinary loads, see ld4c)
+template <int Q, bool CACHED = false>
 __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int z0)
 {
     const int LS = cq_line_stride(nxx);
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     cq_loop<4>(
-        nxx * Q, [&](int e) { return ld4(src + (size_t)(z0 >> 2) * nxx * 4 + (size_t)e * 4); },
+        nxx * Q,
+        [&](int e) {
+            const float CQ_GLOBAL *a = src + (size_t)(z0 >> 2) * nxx * 4 + (size_t)e * 4;
+            return CACHED ? ld4c(a) : ld4(a);
+        },
         [&](int e, float4 d) {
             const int q = cq_div(e, inv), ix = e - q * nxx;
             buf[(2 * q) * LS + ix] = make_float2(d.x, d.y);
@@ -543,12 +548,16 @@ __device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, flo
 //   1: |k|^2                              (k2d1)
 //   2: |k|^(33/16)                        (k2d3)
 //   3: |k|^1.01 = (kx^2+kz^2)^0.505       (k2da)
+// as exp2(a log2 k2) on the hardware log2 / exp2 (v_log_f32, v_exp_f32, about
+// 1 ulp): k2 is 0 (log2 -inf, symbol 0) or at least the smallest nonzero
+// wavenumber squared, never denormal, so the library wrappers' denormal
+// scaling is not needed.
 __device__ __forceinline__ float frac_symbol(int which, float k2)
 {
     if (which == 1) return k2;
-    const float l = log2f(k2);
-    if (which == 3) return exp2f(0.505f * l);
-    const float r = exp2f(l * (1.0f / 32.0f));
+    const float l = __builtin_amdgcn_logf(k2);
+    if (which == 3) return __builtin_amdgcn_exp2f(0.505f * l);
+    const float r = __builtin_amdgcn_exp2f(l * (1.0f / 32.0f));
     return which == 0 ? r : k2 * r;
 }
 
@@ -582,8 +591,9 @@ template <int TK>
 __device__ __forceinline__ void krow_round1(const JobG &J, float2 *buf, int k0, int which, float tt)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
+        const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
         const float k2 = krow_k2(J, kx, k0 + l);
         const float ms = aux_symbol_of(J, which, k2, tt) * J.invN, mk = k2 * J.invN;
         const float2 v = buf[pe], w = buf[TK * LS + pe];
@@ -598,8 +608,9 @@ template <int TK>
 __device__ __forceinline__ void krow_round2(const JobG &J, float2 *buf, int k0)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
+        const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
         const float k2 = krow_k2(J, kx, k0 + l);
         const float r = frac_symbol(0, k2);
         const float ma = r * J.invN, mb = (k2 * r) * J.invN;
@@ -614,8 +625,9 @@ template <int TK>
 __device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst, int k0, int which)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
+        const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
         const float m = frac_symbol(which, krow_k2(J, kx, k0 + l)) * J.invN;
         const float2 v = buf[pe];
         buf[dst * LS + pe] = make_float2(v.x * m, v.y * m);
@@ -627,8 +639,9 @@ template <int TK>
 __device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, int which, float tt = 0.f)
 {
     const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
-        const int l = e / nxx, kx = e - l * nxx, pe = l * LS + kx;
+        const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
         const float m = aux_symbol_of(J, which, krow_k2(J, kx, k0 + l), tt) * J.invN;
         const float2 v = buf[pe];
         buf[pe] = make_float2(v.x * m, v.y * m);
@@ -703,7 +716,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD 
     float2 *const lds = cq_lds;
     float2 *tw = lds, *buf = lds + nxx;
     load_tw(tw, J.twx, nxx);
-    row_fill<Q>(buf, nxx, J.div, z0);
+    row_fill<Q, true>(buf, nxx, J.div, z0);
     __syncthreads();
     cq_fft_fwd(Jr.px, buf, LS, L, tw, J.mx2);
     cq_fft_inv(Jr.px, buf, LS, L, tw);
@@ -724,7 +737,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD 
             r.b = ld4(J.c1 + g);
             r.c = ld4(J.c2 + g);
             r.d = ld4(J.lz + g);
-            r.e = ld4(J.div + g);
+            r.e = ld4c(J.div + g);
             return r;
         },
         [&](int e, const F4x5 &r) {
@@ -965,8 +978,9 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
                     }
                 });
             __syncthreads();
+            const float inv_z = __builtin_amdgcn_rcpf((float)nzz);
             for (int e = threadIdx.x; e < NP * nzz; e += CQ_NT) {
-                const int pr = e / nzz, z = e - pr * nzz;
+                const int pr = cq_div(e, inv_z), z = e - pr * nzz;
                 buf[2 * pr * LS + z].y = buf[(2 * pr + 1) * LS + z].x;
             }
             __syncthreads();
