@@ -49,8 +49,12 @@ __device__ __forceinline__ void f4set(float4 &v, int k, float s)
 // The loop e = threadIdx.x, + CQ_NT, ... < total in groups of U iterations:
 // pre(e) of all U first (the global loads; it returns what it loaded), then
 // post(e, loaded) of all U, so that U iterations' loads are in flight
-// together rather than one iteration's at a time.
-template <int U, typename Pre, typename Post>
+// together rather than one iteration's at a time.  CLAMP: iterations past
+// the end load element total - 1 again (a cached duplicate) instead of
+// branching round their loads, which lets the compiler keep all U in flight
+// where the branches made it wait early (the column passes); their post is
+// skipped as before.
+template <int U, bool CLAMP = false, typename Pre, typename Post>
 __device__ __forceinline__ void cq_loop(int total, Pre pre, Post post)
 {
     using T = decltype(pre(0));
@@ -58,8 +62,10 @@ __device__ __forceinline__ void cq_loop(int total, Pre pre, Post post)
     for (int e0 = threadIdx.x; e0 < total; e0 += U * CQ_NT) {
         T d[U];
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            if (e0 + u * CQ_NT < total) d[u] = pre(e0 + u * CQ_NT);
+        for (int u = 0; u < U; u++) {
+            if (CLAMP) d[u] = pre(min(e0 + u * CQ_NT, total - 1));
+            else if (e0 + u * CQ_NT < total) d[u] = pre(e0 + u * CQ_NT);
+        }
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (e0 + u * CQ_NT < total) post(e0 + u * CQ_NT, d[u]);
@@ -214,7 +220,7 @@ template <int CW>
 __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float CQ_GLOBAL *__restrict__ src, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2;
-    cq_loop<4>(
+    cq_loop<4, true>(
         CW * nq,
         [&](int e) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
@@ -247,7 +253,7 @@ template <int CW>
 __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
-    cq_loop<4>(
+    cq_loop<4, true>(
         CW * nq,
         [&](int e) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
@@ -298,7 +304,7 @@ template <int CW, bool TO_DVZ = false>
 __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2, pml = J.pml;
-    cq_loop<4>(
+    cq_loop<4, true>(
         CW * nq,
         [&](int e) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
@@ -356,7 +362,7 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
     // lanes run over the column pairs of one kz, then the next kz
-    cq_loop<4>(
+    cq_loop<4, true>(
         L * nzh,
         [&](int e) {
             float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -395,19 +401,21 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    cq_loop<4>(
+    cq_loop<4, true>(
         L * nzh,
         [&](int e) {
             const int k = e / L, l = e - k * L;
             const int ix0 = x0 + 2 * l;
             float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
-            if (ix0 + 1 >= J.nxx) h.z = h.w = 0.f;
             return h;
         },
         [&](int e, float4 h) {
             const int k = e / L, l = e - k * L;
             const int k2 = k ? nzz - k : 0;
+            // (masked here rather than at the load, which would make each
+            // load wait for itself)
+            if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;
             float2 h0 = make_float2(h.x, h.y), h1 = make_float2(h.z, h.w);
             float2 *line = buf + l * cq_line_stride(nzz);
             if (k2 == k) {
@@ -443,7 +451,7 @@ __device__ __forceinline__ void col2_insert(const JobG &J, float2 *buf, int LS, 
 {
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
     const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NC));   // opaque: see k_q_c2
-    cq_loop<4>(
+    cq_loop<4, true>(
         NC * nzh,
         [&](int e) {
             const int k = cq_div(e, inv), ix = x0 + (e - k * NC);
@@ -468,17 +476,17 @@ __device__ __forceinline__ void colpair_insert(const JobG &J, float2 *buf, int L
 {
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
     const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NP));
-    cq_loop<4>(
+    cq_loop<4, true>(
         NP * nzh,
         [&](int e) {
             const int k = cq_div(e, inv), ix0 = x0 + 2 * (e - k * NP);
             float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
-            if (ix0 + 1 >= J.nxx) h.z = h.w = 0.f;
             return h;
         },
         [&](int e, float4 h) {
             const int k = cq_div(e, inv), l = e - k * NP;
+            if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;   // as in col_insert
             pack_pair(buf + l * LS, k, k ? nzz - k : 0, make_float2(h.x, h.y), make_float2(h.z, h.w));
         });
 }
@@ -949,7 +957,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             col2_insert<NC>(J, buf, LS, xh, J.s3, J.s4);   // K + i B
             __syncthreads();
             cq_fft_inv(Jr.pz, buf, LS, NC, tw);
-            cq_loop<2>(
+            cq_loop<2, true>(
                 NC * nq,
                 [&](int e) {
                     const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
@@ -987,7 +995,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             colpair_insert<NP>(J, buf + LS, 2 * LS, xh, J.s1);   // A
             __syncthreads();
             cq_fft_inv(Jr.pz, buf + LS, 2 * LS, NP, tw);
-            cq_loop<4>(
+            cq_loop<4, true>(
                 NC * nq,
                 [&](int e) {
                     const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
@@ -1034,7 +1042,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         c = e - zq * CW;
         return x0 + c < J.nxx;
     };
-    cq_loop<4>(
+    cq_loop<4, true>(
         CW * nq,
         [&](int e) {
             int zq, c;
@@ -1058,7 +1066,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     col_insert<CW>(J, buf, x0, J.s4);   // B
     __syncthreads();
     cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
-    cq_loop<4>(
+    cq_loop<4, true>(
         CW * nq,
         [&](int e) {
             int zq, c;
@@ -1088,7 +1096,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     col_insert<CW>(J, buf, x0, J.s1);   // A
     __syncthreads();
     cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
-    cq_loop<3>(
+    cq_loop<3, true>(
         CW * nq,
         [&](int e) {
             int zq, c;
@@ -1189,7 +1197,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     struct C3In {
         float4 p, tot, qa, qb, qe, tt;
     };
-    cq_loop<2>(
+    cq_loop<2, true>(
         CW * nq,
         [&](int e) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
