@@ -223,14 +223,13 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float
     cq_loop<4, true>(
         CW * nq,
         [&](int e) {
-            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ix < J.nxx) v = ld4(src + cq_q4(J.nxx, ix, zq));
-            return v;
+            const int zq = e / CW, c = e - zq * CW;
+            return ld4(src + cq_q4(J.nxx, min(x0 + c, J.nxx - 1), zq));
         },
         [&](int e, float4 v) {
             const int zq = e / CW, c = e - zq * CW;
             float *line = col_line(bufs, nzz, c);
+            if (x0 + c >= J.nxx) v = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = f4get(v, k);
@@ -256,9 +255,9 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
     cq_loop<4, true>(
         CW * nq,
         [&](int e) {
-            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            // clamped column, masked in the second lambda (col_insert)
+            const int zq = e / CW, c = e - zq * CW, ix = min(x0 + c, J.nxx - 1);
             F4x3 r;
-            if (ix >= J.nxx) return r;
             const size_t g = cq_q4(J.nxx, ix, zq);
             r.a = ld4(J.vz + g);
             r.b = ld4(J.dtbz + g);
@@ -307,10 +306,10 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
     cq_loop<4, true>(
         CW * nq,
         [&](int e) {
-            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            // clamped column, masked in the second lambda (col_insert)
+            const int zq = e / CW, c = e - zq * CW, ix = min(x0 + c, J.nxx - 1);
             F4x2 r;
             r.a = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ix >= J.nxx) return r;
             if (!TO_DVZ) r.a = ld4(J.dvx + cq_q4(J.nxx, ix, zq));
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -405,16 +404,16 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
         L * nzh,
         [&](int e) {
             const int k = e / L, l = e - k * L;
-            const int ix0 = x0 + 2 * l;
-            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
-            return h;
+            const int ix0 = min(x0 + 2 * l, (J.nxx - 1) & ~1);
+            return *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
         },
         [&](int e, float4 h) {
             const int k = e / L, l = e - k * L;
             const int k2 = k ? nzz - k : 0;
-            // (masked here rather than at the load, which would make each
-            // load wait for itself)
+            // columns past nxx are masked here rather than branched round at
+            // the load (the loads are issued unconditionally from a clamped
+            // column, so that all of cq_loop's batch stays in flight)
+            if (x0 + 2 * l >= J.nxx) h = make_float4(0.f, 0.f, 0.f, 0.f);
             if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;
             float2 h0 = make_float2(h.x, h.y), h1 = make_float2(h.z, h.w);
             float2 *line = buf + l * cq_line_stride(nzz);
@@ -454,16 +453,13 @@ __device__ __forceinline__ void col2_insert(const JobG &J, float2 *buf, int LS, 
     cq_loop<4, true>(
         NC * nzh,
         [&](int e) {
-            const int k = cq_div(e, inv), ix = x0 + (e - k * NC);
-            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ix < J.nxx) {
-                const float2 a = Sa[cq_hs(ldk, ix, k)], b = Sb[cq_hs(ldk, ix, k)];
-                h = make_float4(a.x, a.y, b.x, b.y);
-            }
-            return h;
+            const int k = cq_div(e, inv), ix = min(x0 + (e - k * NC), J.nxx - 1);   // as in col_insert
+            const float2 a = Sa[cq_hs(ldk, ix, k)], b = Sb[cq_hs(ldk, ix, k)];
+            return make_float4(a.x, a.y, b.x, b.y);
         },
         [&](int e, float4 h) {
             const int k = cq_div(e, inv), j = e - k * NC;
+            if (x0 + j >= J.nxx) h = make_float4(0.f, 0.f, 0.f, 0.f);
             pack_pair(buf + j * LS, k, k ? nzz - k : 0, make_float2(h.x, h.y), make_float2(h.z, h.w));
         });
 }
@@ -479,14 +475,13 @@ __device__ __forceinline__ void colpair_insert(const JobG &J, float2 *buf, int L
     cq_loop<4, true>(
         NP * nzh,
         [&](int e) {
-            const int k = cq_div(e, inv), ix0 = x0 + 2 * (e - k * NP);
-            float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ix0 < J.nxx) h = *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
-            return h;
+            const int k = cq_div(e, inv), ix0 = min(x0 + 2 * (e - k * NP), (J.nxx - 1) & ~1);
+            return *reinterpret_cast<const float4 CQ_GLOBAL *>(S + cq_hs(ldk, ix0, k));
         },
         [&](int e, float4 h) {
             const int k = cq_div(e, inv), l = e - k * NP;
-            if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;   // as in col_insert
+            if (x0 + 2 * l >= J.nxx) h = make_float4(0.f, 0.f, 0.f, 0.f);   // as in col_insert
+            if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;
             pack_pair(buf + l * LS, k, k ? nzz - k : 0, make_float2(h.x, h.y), make_float2(h.z, h.w));
         });
 }
@@ -960,15 +955,14 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             cq_loop<2, true>(
                 NC * nq,
                 [&](int e) {
-                    const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
+                    // clamped column, masked in the second lambda (col_insert)
+                    const int zq = cq_div(e, inv_nc), ix = min(xh + (e - zq * NC), J.nxx - 1);
+                    const size_t g = cq_q4(J.nxx, ix, zq);
                     F4x5 r;
-                    if (ix < J.nxx) {
-                        const size_t g = cq_q4(J.nxx, ix, zq);
-                        r.a = ld4c(J.qg + g);
-                        r.b = ld4c(J.qy2 + g);
-                        r.c = ld4c(J.qcc + g);
-                        r.d = ld4c(J.div + g);
-                    }
+                    r.a = ld4c(J.qg + g);
+                    r.b = ld4c(J.qy2 + g);
+                    r.c = ld4c(J.qcc + g);
+                    r.d = ld4c(J.div + g);
                     return r;
                 },
                 [&](int e, const F4x5 &r) {
@@ -998,10 +992,8 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             cq_loop<4, true>(
                 NC * nq,
                 [&](int e) {
-                    const int zq = cq_div(e, inv_nc), ix = xh + (e - zq * NC);
-                    float4 y2 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (ix < J.nxx) y2 = ld4c(J.qy2 + cq_q4(J.nxx, ix, zq));
-                    return y2;
+                    const int zq = cq_div(e, inv_nc), ix = min(xh + (e - zq * NC), J.nxx - 1);
+                    return ld4c(J.qy2 + cq_q4(J.nxx, ix, zq));
                 },
                 [&](int e, float4 y2) {
                     const int zq = cq_div(e, inv_nc), j = e - zq * NC, ix = xh + j;
@@ -1200,17 +1192,16 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     cq_loop<2, true>(
         CW * nq,
         [&](int e) {
-            const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+            // clamped column, masked in the second lambda (col_insert)
+            const int zq = e / CW, c = e - zq * CW;
+            const size_t g = cq_q4(J.nxx, min(x0 + c, J.nxx - 1), zq);
             C3In r;
-            if (ix < J.nxx) {
-                const size_t g = cq_q4(J.nxx, ix, zq);
-                r.p = ld4(J.p + g);
-                r.tot = ld4(J.tot + g);
-                r.qa = ld4(J.qa + g);
-                r.qb = ld4(J.qb + g);
-                r.qe = ld4(J.qe + g);
-                r.tt = ld4c(J.scr + g);
-            }
+            r.p = ld4(J.p + g);
+            r.tot = ld4(J.tot + g);
+            r.qa = ld4(J.qa + g);
+            r.qb = ld4(J.qb + g);
+            r.qe = ld4(J.qe + g);
+            r.tt = ld4c(J.scr + g);
             return r;
         },
         [&](int e, const C3In &r) {
