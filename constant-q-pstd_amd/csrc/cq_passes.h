@@ -50,7 +50,8 @@ __device__ __forceinline__ void f4set(float4 &v, int k, float s)
 // pre(e) of all U first (the global loads; it returns what it loaded), then
 // post(e, loaded) of all U, so that U iterations' loads are in flight
 // together rather than one iteration's at a time.  CLAMP: iterations past
-// the end load element total - 1 again (a cached duplicate) instead of
+// the end load element total - 1 again (the same address as another lane's
+// load of the batch, or one already read) instead of
 // branching round their loads, which lets the compiler keep all U in flight
 // where the branches made it wait early (the column passes); their post is
 // skipped as before.
