@@ -252,7 +252,7 @@ extern __shared__ float2 cq_lds[];
 // MODE : 0 values in order on both sides; 1 first stage of several, R = pad
 //        period, outputs in the padded layout; 2 second stage after a MODE 1
 //        stage (Ns = pad period), inputs in the padded layout
-template <int R, bool INV, int MODE, int CN = 0, int CNS = 0>
+template <int R, bool INV, int MODE, int CN = 0, int CNS = 0, int CLSM = 1>
 __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
                                       const float2 CQ_GLOBAL *__restrict__ mh)
 {
@@ -264,13 +264,13 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     L = __builtin_amdgcn_readfirstlane(L);
     Ns = __builtin_amdgcn_readfirstlane(Ns);
     to = __builtin_amdgcn_readfirstlane(to);
-    // a stage compiled for one length (CN, lines cq_line_stride(CN) apart,
-    // CNS = Ns): strides and offsets become constants, so the LDS accesses
-    // take immediate offsets instead of one address add each
+    // a stage compiled for one length (CN, lines CLSM cq_line_stride(CN)
+    // apart, CNS = Ns): strides and offsets become constants, so the LDS
+    // accesses take immediate offsets instead of one address add each
     if constexpr (CN > 0) {
         n = CN;
         Ns = CNS;
-        LS = cq_line_stride(CN);
+        LS = CLSM * cq_line_stride(CN);
     }
     float2 *const buf = cq_lds + bo;
     const float2 *const tw = cq_lds + to;
@@ -374,34 +374,41 @@ __device__ __forceinline__ int cq_first_mode(const CqPlan &p)
 // Forward transform of L lines at buf, the plan's twiddle table at tw (both
 // inside cq_lds), optional multiplier fused into the last stage.
 // The three stages of a plan R0 R1 R2 of length N (R0 its pad period) compiled
-// for that length.
-template <bool INV, int N, int R0, int R1, int R2>
+// for that length, lines LSM cq_line_stride(N) apart.
+template <bool INV, int N, int R0, int R1, int R2, int LSM>
 __device__ __forceinline__ void cq_fft3(int bo, int L, int to, const float2 CQ_GLOBAL *__restrict__ mh)
 {
     static_assert(R0 * R1 * R2 == N && R0 == cq_pad_period(N), "plan");
-    constexpr int LS = cq_line_stride(N);
-    cq_stage<R0, INV, 1, N, 1>(bo, LS, N, L, 1, to, nullptr);
-    cq_stage<R1, INV, 2, N, R0>(bo, LS, N, L, R0, to + (R0 - 1), nullptr);
+    constexpr int LS = LSM * cq_line_stride(N);
+    cq_stage<R0, INV, 1, N, 1, LSM>(bo, LS, N, L, 1, to, nullptr);
+    cq_stage<R1, INV, 2, N, R0, LSM>(bo, LS, N, L, R0, to + (R0 - 1), nullptr);
     constexpr int to2 = (R0 - 1) + (R1 - 1) * R0;
     // a forward radix-17 last stage with its multiplier loads at constant
     // offsets issues them all at once and spills: it keeps the generic form
     if constexpr (!INV && R2 == 17) cq_stage<R2, INV, 0>(bo, LS, N, L, R0 * R1, to + to2, mh);
-    else cq_stage<R2, INV, 0, N, R0 * R1>(bo, LS, N, L, R0 * R1, to + to2, mh);
+    else cq_stage<R2, INV, 0, N, R0 * R1, LSM>(bo, LS, N, L, R0 * R1, to + to2, mh);
 }
 
-// Lengths with a compiled plan (the windows of the batched workload): true
-// if the transform ran.
+// Lengths with a compiled plan (the windows of the batched workload; lines
+// one or, for k_q_c2's interleaved lines, two line strides apart): true if
+// the transform ran.
 template <bool INV>
 __device__ __forceinline__ bool cq_fft_fixed(const CqPlan &p, int LS, int bo, int L, int to,
                                              const float2 CQ_GLOBAL *__restrict__ mh)
 {
     if (p.ns != 3) return false;
-    if (p.n == 680 && LS == cq_line_stride(680) && p.rad[0] == 8 && p.rad[1] == 5 && p.rad[2] == 17) {
-        cq_fft3<INV, 680, 8, 5, 17>(bo, L, to, mh);
-        return true;
+    if (p.n == 680 && p.rad[0] == 8 && p.rad[1] == 5 && p.rad[2] == 17) {
+        if (LS == cq_line_stride(680)) {
+            cq_fft3<INV, 680, 8, 5, 17, 1>(bo, L, to, mh);
+            return true;
+        }
+        if (LS == 2 * cq_line_stride(680)) {
+            cq_fft3<INV, 680, 8, 5, 17, 2>(bo, L, to, mh);
+            return true;
+        }
     }
     if (p.n == 1296 && LS == cq_line_stride(1296) && p.rad[0] == 16 && p.rad[1] == 9 && p.rad[2] == 9) {
-        cq_fft3<INV, 1296, 16, 9, 9>(bo, L, to, mh);
+        cq_fft3<INV, 1296, 16, 9, 9, 1>(bo, L, to, mh);
         return true;
     }
     return false;
