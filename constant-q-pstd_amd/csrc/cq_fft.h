@@ -252,7 +252,7 @@ extern __shared__ float2 cq_lds[];
 // MODE : 0 values in order on both sides; 1 first stage of several, R = pad
 //        period, outputs in the padded layout; 2 second stage after a MODE 1
 //        stage (Ns = pad period), inputs in the padded layout
-template <int R, bool INV, int MODE, int CN = 0, int CNS = 0, int CLSM = 1>
+template <int R, bool INV, int MODE, int CN = 0, int CNS = 0, int CLSM = 1, bool MH = true>
 __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
                                       const float2 CQ_GLOBAL *__restrict__ mh)
 {
@@ -315,7 +315,7 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
             // k = 0), j (R + 1) + q
             kk[b] = (j - k) * R + k;
             base[b] = l * LS + (MODE == 1 ? j * (R + 1) : kk[b]);
-            if (mh != nullptr) {
+            if (MH && mh != nullptr) {
 #pragma unroll
                 for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk[b] + q * Ns]);
             }
@@ -383,10 +383,15 @@ __device__ __forceinline__ void cq_fft3(int bo, int L, int to, const float2 CQ_G
     cq_stage<R0, INV, 1, N, 1, LSM>(bo, LS, N, L, 1, to, nullptr);
     cq_stage<R1, INV, 2, N, R0, LSM>(bo, LS, N, L, R0, to + (R0 - 1), nullptr);
     constexpr int to2 = (R0 - 1) + (R1 - 1) * R0;
-    // a forward radix-17 last stage with its multiplier loads at constant
-    // offsets issues them all at once and spills: it keeps the generic form
-    if constexpr (!INV && R2 == 17) cq_stage<R2, INV, 0>(bo, LS, N, L, R0 * R1, to + to2, mh);
-    else cq_stage<R2, INV, 0, N, R0 * R1, LSM>(bo, LS, N, L, R0 * R1, to + to2, mh);
+    // compiled for the length with its multiplier code, a forward radix-17
+    // last stage needs 120 VGPRs and spills: with a multiplier it keeps the
+    // generic form, without one it is compiled without that code
+    if constexpr (!INV && R2 == 17) {
+        if (mh != nullptr) cq_stage<R2, INV, 0>(bo, LS, N, L, R0 * R1, to + to2, mh);
+        else cq_stage<R2, INV, 0, N, R0 * R1, LSM, false>(bo, LS, N, L, R0 * R1, to + to2, nullptr);
+    } else {
+        cq_stage<R2, INV, 0, N, R0 * R1, LSM>(bo, LS, N, L, R0 * R1, to + to2, mh);
+    }
 }
 
 // Lengths with a compiled plan (the windows of the batched workload; lines
