@@ -373,50 +373,60 @@ __device__ __forceinline__ int cq_first_mode(const CqPlan &p)
 
 // Forward transform of L lines at buf, the plan's twiddle table at tw (both
 // inside cq_lds), optional multiplier fused into the last stage.
-// The three stages of a plan R0 R1 R2 of length N (R0 its pad period) compiled
-// for that length, lines LSM cq_line_stride(N) apart.
-template <bool INV, int N, int R0, int R1, int R2, int LSM>
-__device__ __forceinline__ void cq_fft3(int bo, int L, int to, const float2 CQ_GLOBAL *__restrict__ mh)
+// The stages R, RS... of a plan of length N compiled for that length, lines
+// LSM cq_line_stride(N) apart; MODE, NS and TO are the current stage's layout
+// mode, Ns and twiddle offset (the generic loop's run-time values).
+template <bool INV, int N, int LSM, int MODE, int NS, int TO, int R, int... RS>
+__device__ __forceinline__ void cq_fft_stages(int bo, int L, int to, const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    static_assert(R0 * R1 * R2 == N && R0 == cq_pad_period(N), "plan");
     constexpr int LS = LSM * cq_line_stride(N);
-    cq_stage<R0, INV, 1, N, 1, LSM>(bo, LS, N, L, 1, to, nullptr);
-    cq_stage<R1, INV, 2, N, R0, LSM>(bo, LS, N, L, R0, to + (R0 - 1), nullptr);
-    constexpr int to2 = (R0 - 1) + (R1 - 1) * R0;
-    // compiled for the length with its multiplier code, a forward radix-17
-    // last stage needs 120 VGPRs and spills: with a multiplier it keeps the
-    // generic form, without one it is compiled without that code
-    if constexpr (!INV && R2 == 17) {
-        if (mh != nullptr) cq_stage<R2, INV, 0>(bo, LS, N, L, R0 * R1, to + to2, mh);
-        else cq_stage<R2, INV, 0, N, R0 * R1, LSM, false>(bo, LS, N, L, R0 * R1, to + to2, nullptr);
+    if constexpr (sizeof...(RS) > 0) {
+        cq_stage<R, INV, MODE, N, NS, LSM>(bo, LS, N, L, NS, to + TO, nullptr);
+        cq_fft_stages<INV, N, LSM, (MODE == 1 ? 2 : 0), NS * R, TO + (R - 1) * NS, RS...>(bo, L, to, mh);
+    } else if constexpr (!INV && R == 17) {
+        // compiled for the length with its multiplier code, a forward
+        // radix-17 last stage needs 120 VGPRs and spills: with a multiplier
+        // it keeps the generic form, without one it is compiled without
+        // that code
+        if (mh != nullptr) cq_stage<R, INV, MODE>(bo, LS, N, L, NS, to + TO, mh);
+        else cq_stage<R, INV, MODE, N, NS, LSM, false>(bo, LS, N, L, NS, to + TO, nullptr);
     } else {
-        cq_stage<R2, INV, 0, N, R0 * R1, LSM>(bo, LS, N, L, R0 * R1, to + to2, mh);
+        cq_stage<R, INV, MODE, N, NS, LSM>(bo, LS, N, L, NS, to + TO, mh);
     }
 }
 
-// Lengths with a compiled plan (the windows of the batched workload; lines
-// one or, for k_q_c2's interleaved lines, two line strides apart): true if
-// the transform ran.
+// A plan R0 RS... of length N as compiled stages, if p is that plan and its
+// lines are LSM line strides apart; the first stage's mode follows
+// cq_first_mode.
+template <bool INV, int N, int LSM, int R0, int... RS>
+__device__ __forceinline__ bool cq_fft_plan(const CqPlan &p, int LS, int bo, int L, int to,
+                                            const float2 CQ_GLOBAL *__restrict__ mh)
+{
+    constexpr int rad[] = {R0, RS...};
+    constexpr int ns = 1 + sizeof...(RS);
+    static_assert((R0 * ... * RS) == N, "plan");
+    if (p.n != N || p.ns != ns || LS != LSM * cq_line_stride(N)) return false;
+    for (int s = 0; s < ns; s++)
+        if (p.rad[s] != rad[s]) return false;
+    constexpr int mode0 = ns > 1 && R0 == cq_pad_period(N) ? 1 : 0;
+    cq_fft_stages<INV, N, LSM, mode0, 1, 0, R0, RS...>(bo, L, to, mh);
+    return true;
+}
+
+// Lengths with compiled plans: the windows of the batched workload (680 deep,
+// 1296 wide) and of the Q-RTM workload (880 deep, 2100 wide); lines one or,
+// for k_q_c2's interleaved lines, two line strides apart.  True if the
+// transform ran.
 template <bool INV>
 __device__ __forceinline__ bool cq_fft_fixed(const CqPlan &p, int LS, int bo, int L, int to,
                                              const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    if (p.ns != 3) return false;
-    if (p.n == 680 && p.rad[0] == 8 && p.rad[1] == 5 && p.rad[2] == 17) {
-        if (LS == cq_line_stride(680)) {
-            cq_fft3<INV, 680, 8, 5, 17, 1>(bo, L, to, mh);
-            return true;
-        }
-        if (LS == 2 * cq_line_stride(680)) {
-            cq_fft3<INV, 680, 8, 5, 17, 2>(bo, L, to, mh);
-            return true;
-        }
-    }
-    if (p.n == 1296 && LS == cq_line_stride(1296) && p.rad[0] == 16 && p.rad[1] == 9 && p.rad[2] == 9) {
-        cq_fft3<INV, 1296, 16, 9, 9, 1>(bo, L, to, mh);
-        return true;
-    }
-    return false;
+    return cq_fft_plan<INV, 680, 1, 8, 5, 17>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, 680, 2, 8, 5, 17>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, 1296, 1, 16, 9, 9>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, 880, 1, 16, 5, 11>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, 880, 2, 16, 5, 11>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, 2100, 1, 4, 3, 5, 5, 7>(p, LS, bo, L, to, mh);
 }
 
 __device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,

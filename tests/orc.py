@@ -59,6 +59,13 @@ class _Args:
     def __init__(self, job, bits):
         dt = _dt(bits)
         self.keep = []
+        # the oracle dereferences these without checks: a missing one would
+        # crash it, so refuse the job here
+        q = int(job["kind"]) in (1, 3)
+        need = FIELD_KEYS[:8 if q else 4] + C1D_KEYS + (K2_KEYS if q else K2_KEYS[:1])
+        missing = [k for k in need if job.get(k) is None]
+        if missing:
+            raise ValueError("job lacks %s (build it with the k powers: want_k2=True)" % ", ".join(missing))
         PT = C.POINTER(C.c_double if bits == 64 else C.c_float)
 
         def arr(keys):

@@ -173,6 +173,33 @@ def test_tall_window_vs_oracle(ctx, kind):
     assert e <= TOL
 
 
+def _window(kind, nx, nz, seed):
+    """A shot on an explicit nx x nz window (10-point CPML) of a layered
+    model, receivers on every fourth surface column."""
+    dx, pml = 10.0, 10
+    vp, rho, q = synth.layered_model(nx, nz, dx, seed=seed, nlayers=3, qrange=(20.0, 200.0))
+    return synth.make_window_job(kind, vp, rho, nx, nz, dx, dx, 0.001, 30, 25.0, 0, nx, nx // 2, 2,
+                                 np.arange(0, nx, 4), 2, pml, qt=q, fo=50.0, want_k2=True)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("shape", [(2080, 40), (45, 860)])
+def test_qrtm_window_lengths_vs_oracle(ctx, kind, shape):
+    """The lengths of the Q-RTM workload's windows, which run FFT stages
+    compiled for them: 2100 = 4*3*5*5*7 wide (row and kz-row transforms) and
+    880 = 16*5*11 deep (column transforms)."""
+    job = _window(kind, shape[0], shape[1], seed=41 + kind)
+    print("case Q-RTM lengths kind", kind, job["nxx"], job["nzz"], flush=True)
+    assert (job["nxx"], job["nzz"]) in ((2100, 60), (65, 880))
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_benchmark_window_vs_oracle(ctx, kind):
     """One interior shot of the batched benchmark workload at its full
