@@ -102,7 +102,7 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GL
 {
     const int LS = cq_line_stride(nxx);
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
-    cq_loop<4>(
+    cq_loop<8>(
         nxx * Q,
         [&](int e) {
             const float CQ_GLOBAL *a = src + (size_t)(z0 >> 2) * nxx * 4 + (size_t)e * 4;
@@ -221,7 +221,7 @@ template <int CW>
 __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float CQ_GLOBAL *__restrict__ src, int x0)
 {
     const int nzz = J.nzz, nq = (nzz + 3) >> 2;
-    cq_loop<4, true>(
+    cq_loop<8, true>(
         CW * nq,
         [&](int e) {
             const int zq = e / CW, c = e - zq * CW;
@@ -401,7 +401,7 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
 {
     constexpr int L = CW / 2;
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
-    cq_loop<4, true>(
+    cq_loop<8, true>(
         L * nzh,
         [&](int e) {
             const int k = e / L, l = e - k * L;
@@ -451,7 +451,7 @@ __device__ __forceinline__ void col2_insert(const JobG &J, float2 *buf, int LS, 
 {
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
     const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NC));   // opaque: see k_q_c2
-    cq_loop<4, true>(
+    cq_loop<8, true>(
         NC * nzh,
         [&](int e) {
             const int k = cq_div(e, inv), ix = min(x0 + (e - k * NC), J.nxx - 1);   // as in col_insert
@@ -473,7 +473,7 @@ __device__ __forceinline__ void colpair_insert(const JobG &J, float2 *buf, int L
 {
     const int nzz = J.nzz, nzh = J.nzh, ldk = J.ldk;
     const float inv = __builtin_amdgcn_rcpf((float)__builtin_amdgcn_readfirstlane(NP));
-    cq_loop<4, true>(
+    cq_loop<8, true>(
         NP * nzh,
         [&](int e) {
             const int k = cq_div(e, inv), ix0 = min(x0 + 2 * (e - k * NP), (J.nxx - 1) & ~1);
