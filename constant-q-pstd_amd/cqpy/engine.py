@@ -37,7 +37,7 @@ class ServerDesc(C.Structure):
                 ("max_jobs", C.c_int)]
 
 
-QUEUED, RUNNING, DONE = 0, 1, 2
+QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
 SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_create",
            "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",

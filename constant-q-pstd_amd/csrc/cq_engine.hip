@@ -1166,6 +1166,14 @@ static int read_field(const JobH &h, const float *dev, int x0, int nx, int z0, i
     return CQ_OK;
 }
 
+// Index of the first non-finite value of a[0 .. n), or -1.
+static long first_nonfinite(const float *a, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(a[i])) return (long)i;
+    return -1;
+}
+
 static int read_image(cq_ctx ctx, const JobH &h, float *img, float *ssg)
 {
     if (!is_rtm(h.kind)) return set_err(CQ_ESTATE, "not an RTM shot");
@@ -1173,6 +1181,11 @@ static int read_image(cq_ctx ctx, const JobH &h, float *img, float *ssg)
     const int pml = (h.nxx - h.nx) / 2;
     int rc = read_field(h, h.d[0].img, pml, h.nx, pml, h.nz, img);
     if (rc == CQ_OK) rc = read_field(h, h.d[0].ssg, pml, h.nx, pml, h.nz, ssg);
+    if (rc == CQ_OK) {
+        const size_t n = (size_t)h.nx * h.nz;
+        if (first_nonfinite(img, n) >= 0 || first_nonfinite(ssg, n) >= 0)
+            return set_err(CQ_ENUMERIC, "RTM shot: non-finite image or illumination (the wavefield blew up)");
+    }
     return rc;
 }
 
@@ -1180,7 +1193,13 @@ static int read_rec(cq_ctx ctx, const JobH &h, float *rec)
 {
     if (is_rtm(h.kind)) return set_err(CQ_ESTATE, "RTM shots have no gathers");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (h.nr > 0) HIPCHK(hipMemcpy(rec, h.d[0].rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
+    if (h.nr > 0) {
+        HIPCHK(hipMemcpy(rec, h.d[0].rec, sizeof(float) * (size_t)h.nt * h.nr, hipMemcpyDeviceToHost));
+        const long i = first_nonfinite(rec, (size_t)h.nt * h.nr);
+        if (i >= 0)
+            return set_err(CQ_ENUMERIC, "non-finite gather value at step %ld, receiver %ld (the wavefield blew up)",
+                           i / h.nr, i % h.nr);
+    }
     return CQ_OK;
 }
 
@@ -1227,6 +1246,8 @@ struct ServerJob {
     int t_submit = 0, t_start = -1, t_end = -1;
     size_t bytes = 0;
     std::vector<float> out, ssg;   // results (gather, or image + illumination)
+    int err = CQ_OK;               // status of a failed job
+    std::string msg;               // and its reason
 };
 
 struct cq_server_s {
@@ -1237,7 +1258,18 @@ struct cq_server_s {
     std::map<int, std::unique_ptr<ServerJob>> jobs;
     std::vector<int> queue;         // ids waiting for admission, FIFO
     std::vector<int> running;       // ids of b.jobs, same order
+    std::vector<char> blown;        // per running job: non-finite wavefield found
+    int last_check = 0;             // global step of the last non-finite check
+    int *dflag = nullptr;           // device flags of that check, one per wavefield
+    size_t dflag_cap = 0;
+    std::set<int> fail_admit;       // CQ_FAULT_ADMIT: ids whose admission is made to fail
+    ~cq_server_s() { (void)hipFree(dflag); }
 };
+
+// Global steps between two checks of the running jobs' wavefields for
+// non-finite values (one read of each pressure field: 4 B per cell every
+// kCheckEvery steps).
+static const int kCheckEvery = 256;
 
 // Tile shapes fixed for every window up to max_nxx x max_nzz: the largest
 // tiles whose LDS fits the budget and whose point count fits the smallest
@@ -1290,6 +1322,18 @@ extern "C" int cq_server_open(cq_ctx ctx, const cq_server_desc *desc, cq_server 
         size_t fr = 0, tot = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
         s->budget = fr / 10 * 9;
+    }
+    // fault injection for tests of the admission-failure path: a comma list
+    // of job ids (in submission order, from 1) whose admission fails as if
+    // their device allocation had
+    if (const char *fa = getenv("CQ_FAULT_ADMIT")) {
+        for (const char *c = fa; *c;) {
+            char *e = nullptr;
+            const long v = strtol(c, &e, 10);
+            if (e == c) break;
+            s->fail_admit.insert((int)v);
+            c = (*e == ',') ? e + 1 : e;
+        }
     }
     *out = s.release();
     ctx->servers.insert(*out);
@@ -1349,55 +1393,129 @@ extern "C" int cq_server_submit(cq_server s, const cq_job_desc *job, int *id)
     return CQ_OK;
 }
 
-// Copies finished jobs' results to the host and frees their device memory;
-// then admits queued jobs (FIFO) that fit the budget and the job limit.  The
+__global__ __launch_bounds__(256) void k_nonfinite(const float *__restrict__ f, long long n, int *__restrict__ flag)
+{
+    int bad = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bad |= !isfinite(f[i]);
+    if (__syncthreads_or(bad) && threadIdx.x == 0) *flag = 1;
+}
+
+// Marks the running jobs whose pressure field (either wavefield of an RTM
+// shot) holds a non-finite value; server_turnover retires them as failed.
+static int check_nonfinite(cq_server s)
+{
+    cq_batch b = &s->b;
+    s->last_check = b->t;
+    const size_t n = 2 * b->jobs.size();
+    if (n == 0) return CQ_OK;
+    if (n > s->dflag_cap) {
+        (void)hipFree(s->dflag);
+        s->dflag = nullptr;
+        s->dflag_cap = std::max(n, 2 * s->dflag_cap);
+        HIPCHK(hipMalloc(&s->dflag, sizeof(int) * s->dflag_cap));
+    }
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemsetAsync(s->dflag, 0, sizeof(int) * n, st));
+    for (size_t i = 0; i < b->jobs.size(); i++) {
+        const JobH &h = b->jobs[i];
+        for (int r = 0; r < h.nroles; r++) {
+            hipLaunchKernelGGL(k_nonfinite, dim3(64), dim3(256), 0, st, (const float *)h.d[r].p,
+                               (long long)h.nxx * h.ldz, s->dflag + 2 * i + r);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    std::vector<int> f(n);
+    HIPCHK(hipMemcpyAsync(f.data(), s->dflag, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s->blown.assign(b->jobs.size(), 0);
+    for (size_t i = 0; i < b->jobs.size(); i++) s->blown[i] = (char)(f[2 * i] | f[2 * i + 1]);
+    return CQ_OK;
+}
+
+static void fail_job(ServerJob &sj, int err, int t)
+{
+    sj.state = CQ_JOB_FAILED;
+    sj.err = err;
+    sj.msg = g_err;
+    sj.t_end = t;
+    sj.out.clear();
+    sj.ssg.clear();
+    for (auto &v : sj.f) std::vector<float>().swap(v);
+}
+
+// Retires finished jobs (results copied to the host) and jobs whose
+// wavefield blew up (failed, CQ_ENUMERIC), freeing their device memory; then
+// admits queued jobs (FIFO) that fit the budget and the job limit.  A job
+// whose admission fails is marked failed and the next one is tried.  The
 // stream is idle on return when anything changed, and the tables are rebuilt.
+// Only HIP runtime errors are returned.
 static int server_turnover(cq_server s)
 {
     cq_batch b = &s->b;
-    bool changed = false;
+    bool changed = false, synced = false;
+    s->blown.resize(b->jobs.size(), 0);
     for (size_t i = 0; i < b->jobs.size();) {
         JobH &h = b->jobs[i];
-        if (b->t < h.t0 + h.nsched) {
+        const bool blown = s->blown[i] != 0;
+        if (!blown && b->t < h.t0 + h.nsched) {
             i++;
             continue;
         }
         ServerJob &sj = *s->jobs[s->running[i]];
-        if (is_rtm(h.kind)) {
+        int rc = CQ_OK;
+        if (blown) {
+            rc = set_err(CQ_ENUMERIC, "job %d: non-finite wavefield by global step %d (the wavefield blew up)", sj.id,
+                         b->t);
+        } else if (is_rtm(h.kind)) {
             sj.out.resize((size_t)h.nx * h.nz);
             sj.ssg.resize((size_t)h.nx * h.nz);
-            int rc = read_image(b->ctx, h, sj.out.data(), sj.ssg.data());
-            if (rc) return rc;
+            rc = read_image(b->ctx, h, sj.out.data(), sj.ssg.data());
         } else {
             sj.out.resize((size_t)h.nt * h.nr);
-            int rc = read_rec(b->ctx, h, sj.out.data());
-            if (rc) return rc;
+            rc = read_rec(b->ctx, h, sj.out.data());
         }
-        sj.state = CQ_JOB_DONE;
-        sj.t_end = b->t;
+        if (rc == CQ_ENUMERIC) fail_job(sj, rc, b->t);
+        else if (rc) return rc;
+        else {
+            sj.state = CQ_JOB_DONE;
+            sj.t_end = b->t;
+        }
+        if (!synced) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        synced = true;
         HIPCHK(hipFree(h.mem));
         s->resident -= sj.bytes;
         b->jobs.erase(b->jobs.begin() + i);
         s->running.erase(s->running.begin() + i);
+        s->blown.erase(s->blown.begin() + i);
         changed = true;
     }
     while (!s->queue.empty()) {
         ServerJob &sj = *s->jobs[s->queue.front()];
         if (s->resident + sj.bytes > s->budget) break;
         if (s->desc.max_jobs > 0 && (int)b->jobs.size() >= s->desc.max_jobs) break;
-        if (!changed) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        if (!synced) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        synced = true;
         b->jobs.emplace_back();
         JobH &h = b->jobs.back();
-        int rc = build_job(b, sj.d, h);
+        int rc = s->fail_admit.count(sj.id)
+                     ? set_err(CQ_ENOMEM, "job %d: device allocation failed (injected by CQ_FAULT_ADMIT)", sj.id)
+                     : build_job(b, sj.d, h);
+        s->queue.erase(s->queue.begin());
         if (rc) {
+            // the job leaves the queue as failed; what build_job had
+            // allocated is returned, and the next job is tried
+            (void)hipStreamSynchronize(b->ctx->stream);
+            if (h.mem) (void)hipFree(h.mem);
             b->jobs.pop_back();
-            return rc;
+            fail_job(sj, rc, b->t);
+            continue;
         }
         h.t0 = b->t;
         for (int r = 0; r < h.nroles; r++) h.d[r].t0 = b->t;
         s->resident += sj.bytes;
         s->running.push_back(sj.id);
-        s->queue.erase(s->queue.begin());
+        s->blown.push_back(0);
         sj.state = CQ_JOB_RUNNING;
         sj.t_start = b->t;
         for (auto &v : sj.f) std::vector<float>().swap(v);   // inputs now live on the device
@@ -1415,10 +1533,12 @@ extern "C" int cq_server_advance(cq_server s, int nsteps, int *running, int *que
     const int end = b->t + nsteps;
     int rc = server_turnover(s);
     while (rc == CQ_OK && b->t < end && !b->jobs.empty()) {
-        // run to the next finishing job (or the end), then turn over
-        int next = end;
+        // run to the next finishing job, non-finite check or the end, then
+        // turn over
+        int next = std::min(end, s->last_check + kCheckEvery);
         for (auto &h : b->jobs) next = std::min(next, h.t0 + h.nsched);
         rc = advance_to(b, std::max(next, b->t + 1));
+        if (rc == CQ_OK && b->t >= s->last_check + kCheckEvery) rc = check_nonfinite(s);
         if (rc == CQ_OK && b->t >= next) {
             HIPCHK(hipStreamSynchronize(b->ctx->stream));
             rc = server_turnover(s);
@@ -1458,6 +1578,7 @@ extern "C" int cq_server_result(cq_server s, int id, float *out, float *ssg)
 {
     ServerJob *sj = s ? server_job(s, id) : nullptr;
     if (!sj || !out) return set_err(CQ_EINVAL, "cq_server_result: bad arguments");
+    if (sj->state == CQ_JOB_FAILED) return set_err(sj->err, "%s", sj->msg.c_str());
     if (sj->state != CQ_JOB_DONE) return set_err(CQ_ESTATE, "job %d has not finished", id);
     memcpy(out, sj->out.data(), sizeof(float) * sj->out.size());
     if (!sj->ssg.empty()) {
@@ -1471,7 +1592,7 @@ extern "C" int cq_server_release(cq_server s, int id)
 {
     ServerJob *sj = s ? server_job(s, id) : nullptr;
     if (!sj) return set_err(CQ_EINVAL, "cq_server_release: unknown job %d", id);
-    if (sj->state != CQ_JOB_DONE) return set_err(CQ_ESTATE, "job %d has not finished", id);
+    if (sj->state != CQ_JOB_DONE && sj->state != CQ_JOB_FAILED) return set_err(CQ_ESTATE, "job %d has not finished", id);
     s->jobs.erase(id);
     return CQ_OK;
 }
@@ -1582,6 +1703,8 @@ extern "C" int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *
 extern "C" int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec)
 {
     if (!ctx || !job || !rec) return set_err(CQ_EINVAL, "cq_plain_forward: bad arguments");
+    if (is_rtm(job->kind))
+        return set_err(CQ_EINVAL, "cq_plain_forward: modeling shots only (kind %d is RTM)", job->kind);
     cq_batch b = nullptr;
     int rc = cq_batch_create(ctx, job, 1, &b);
     if (rc) return rc;
@@ -1599,6 +1722,10 @@ extern "C" int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec)
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) rc = set_err(CQ_EDEVICE, "plain gather copy: %s", hipGetErrorString(e));
             else memcpy(rec + (size_t)t * h.nr, pinned, sizeof(float) * h.nr);
+            const long i = rc == CQ_OK ? first_nonfinite(pinned, (size_t)h.nr) : -1;
+            if (i >= 0)
+                rc = set_err(CQ_ENUMERIC, "non-finite gather value at step %d, receiver %ld (the wavefield blew up)",
+                             t, i);
         }
     }
     if (pinned) (void)hipHostFree(pinned);

@@ -6,13 +6,14 @@
  * sgpstd-qrtm/acrtmain.c (qrtm): the same parameter files, geometry and model
  * files and the same outputs (reca<shot>_<nr>.bin / recva<shot>_<nr>.bin gathers,
  * RTM.BIN / QRTM.BIN = [image | illumination]).  The MPI shot loop becomes
- * on-GPU batching: every shot of the run advances in the same launches
- * (modeling: chunks of `batch` shots; RTM: all shots in lock step).
+ * on-GPU batching: the shots of a chunk of `batch` shots advance in the same
+ * launches (RTM shots each on their own revolve schedule).
  *
  * usage: <aps|vaps|artm|qrtm> [parameter file] [gather directory] [batch]
  *   parameter file   default ../input/<reference name>.txt
  *   gather directory default ../obsdata (modeling output, as acps4.cu:694)
- *   batch            shots per modeling batch, default 64
+ *   batch            shots per batch, default 64 (halved while a batch does
+ *                    not fit the device)
  * The mode is taken from the program name (argv[0]) or CQ_MODE.
  */
 #include <libgen.h>
@@ -162,8 +163,12 @@ int main(int argc, char **argv)
               *kzsmi = (float *)malloc(4 * nzz), *kzsci = (float *)malloc(4 * nzz);
         cqh_shift_coeffs(kxsmr, kxscr, kzsmr, kzscr, kxsmi, kxsci, kzsmi, kzsci, dx, dz, nxx, nzz);
 
-        const int chunk = rtm ? g1 - g0 : (batch > 0 ? batch : 64);
-        for (int s0 = g0; s0 < g1; s0 += chunk) {
+        /* shots run in batches of `chunk`; a batch that does not fit the
+         * device (an RTM shot holds snaps checkpoints of its state) is retried
+         * with half as many shots.  RTM images are stacked on the host per
+         * shot, so nothing needs all shots in one batch. */
+        int chunk = batch > 0 ? batch : 64;
+        for (int s0 = g0; s0 < g1;) {
             const int ns = (s0 + chunk <= g1) ? chunk : g1 - s0;
             shot_buf *sb = (shot_buf *)calloc(ns, sizeof(shot_buf));
             cq_job_desc *jd = (cq_job_desc *)calloc(ns, sizeof(cq_job_desc));
@@ -215,7 +220,16 @@ int main(int argc, char **argv)
                 printf("shotid=%d nxx=%d nzz=%d xs=%f zs=%f\n", sid, nxx, nzz, xs, zs);
             }
             cq_batch bt;
-            if (cq_batch_create(ctx, jd, ns, &bt)) DIE("cq_batch_create: %s", cq_last_error());
+            const int rc = cq_batch_create(ctx, jd, ns, &bt);
+            if (rc == CQ_ENOMEM && ns > 1) {
+                printf(" %d shots do not fit the device (%s); retrying with %d\n", ns, cq_last_error(), (ns + 1) / 2);
+                for (int k = 0; k < ns; k++) free_shot(&sb[k]);
+                free(sb);
+                free(jd);
+                chunk = (ns + 1) / 2;
+                continue;
+            }
+            if (rc) DIE("cq_batch_create: %s", cq_last_error());
             if (!rtm && P.flagout == 1) {
                 /* snapshots (sgpstd-qmodeling/acps4.cu:482-494, 624-640): the
                  * interior of p after every step t with t % 10 == 0, appended
@@ -251,7 +265,7 @@ int main(int argc, char **argv)
                 if (!rtm) {
                     char name[512];
                     float *rec = (float *)malloc(sizeof(float) * (size_t)tmax * (b->nr > 0 ? b->nr : 1));
-                    if (cq_batch_read_rec(bt, k, rec)) DIE("cq_batch_read_rec: %s", cq_last_error());
+                    if (cq_batch_read_rec(bt, k, rec)) DIE("shot %d: %s", b->shotid, cq_last_error());
                     snprintf(name, sizeof name, "%s/%s%d_%d.bin", outdir, q ? "recva" : "reca", b->shotid, b->nr);
                     if (cqh_write_f32(name, rec, (long)tmax * b->nr)) DIE("cannot write %s", name);
                     free(rec);
@@ -261,7 +275,7 @@ int main(int argc, char **argv)
                     float *imgf = (float *)malloc(sizeof(float) * (size_t)nx * nz);
                     float *ssg = (float *)malloc(sizeof(float) * (size_t)nx * nz);
                     float *pj = (float *)calloc(nzxt, sizeof(float));
-                    if (cq_batch_read_image(bt, k, img, ssg)) DIE("cq_batch_read_image: %s", cq_last_error());
+                    if (cq_batch_read_image(bt, k, img, ssg)) DIE("shot %d: %s", b->shotid, cq_last_error());
                     cqh_filter_image(nx, nz, dx, dz, img, imgf);
                     cqh_mute_image(nx, nz, pml, b->geo.lgest, b->geo.rgest, imgf);
                     cqh_projectomodel(pml, b->geo.mlx, dx, nxt, nzt, nx, nz, nxx, nzz, pj, imgf);
@@ -276,6 +290,7 @@ int main(int argc, char **argv)
             cq_batch_destroy(bt);
             free(sb);
             free(jd);
+            s0 += ns;
         }
         free(iax); free(ibx); free(hax); free(hbx); free(iaz); free(ibz); free(haz); free(hbz);
         free(kxsmr); free(kxscr); free(kxsmi); free(kxsci); free(kzsmr); free(kzscr); free(kzsmi); free(kzsci);

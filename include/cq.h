@@ -98,13 +98,17 @@ int cq_batch_step(cq_batch b, int nsteps);
 int cq_batch_sync(cq_batch b);
 /* Runs every shot to its end and waits. */
 int cq_batch_run(cq_batch b);
-/* Copies modeling shot `job`'s gather (nt*nr floats) to host memory. */
+/* Copies modeling shot `job`'s gather (nt*nr floats) to host memory.
+ * Returns CQ_ENUMERIC (after copying) if the gather holds a non-finite value,
+ * i.e. the shot blew up (for instance a time step beyond the CFL limit; the
+ * reference's own check is disabled, acousticsgps4.c:221-224). */
 int cq_batch_read_rec(cq_batch b, int job, float *rec);
 /* Copies shot `job`'s pressure field (nxx*nzz floats) to host memory. */
 int cq_batch_read_p(cq_batch b, int job, float *p);
 /* RTM shots: copies shot `job`'s raw cross-correlation image and source
  * illumination, nx*nz floats each over the interior (x-major), before the
- * Laplacian filter, mute and projection of sgpstd-qrtm/acps4.cu:909-945. */
+ * Laplacian filter, mute and projection of sgpstd-qrtm/acps4.cu:909-945.
+ * Returns CQ_ENUMERIC (after copying) if either holds a non-finite value. */
 int cq_batch_read_image(cq_batch b, int job, float *img, float *ssg);
 /* Global steps taken so far by the batch. */
 int cq_batch_time(cq_batch b);
@@ -127,7 +131,11 @@ typedef struct {
     int max_jobs;           /* jobs resident at once; 0: no limit */
 } cq_server_desc;
 
-typedef enum { CQ_JOB_QUEUED = 0, CQ_JOB_RUNNING = 1, CQ_JOB_DONE = 2 } cq_job_state;
+/* CQ_JOB_FAILED: the job could not be admitted (e.g. its device allocation
+ * failed) or its wavefield became non-finite (CQ_ENUMERIC, checked every few
+ * hundred global steps and at completion); cq_server_result then returns that
+ * status and cq_last_error() its reason.  The other jobs are not affected. */
+typedef enum { CQ_JOB_QUEUED = 0, CQ_JOB_RUNNING = 1, CQ_JOB_DONE = 2, CQ_JOB_FAILED = 3 } cq_job_state;
 
 int cq_server_open(cq_ctx ctx, const cq_server_desc *desc, cq_server *out);
 /* Validates and copies the job (all arrays); returns its id. */
@@ -146,9 +154,11 @@ int cq_server_result(cq_server s, int id, float *out, float *ssg);
 int cq_server_release(cq_server s, int id);
 int cq_server_close(cq_server s);
 
-/* The plain path: one shot, one launch sequence per time step, the gather
+/* The plain path: one modeling shot (kind ACOUSTIC or VISCO; RTM kinds are
+ * rejected with CQ_EINVAL), one launch sequence per time step, the gather
  * row copied to the host and the stream synchronised after every step (the
- * reference's per-step structure, acps4.cu:508-621).  rec receives nt*nr. */
+ * reference's per-step structure, acps4.cu:508-621).  rec receives nt*nr.
+ * Stops with CQ_ENUMERIC at the first gather row holding a non-finite value. */
 int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec);
 
 /* Per-pass device timing with HIP events on the engine's stream.  While
