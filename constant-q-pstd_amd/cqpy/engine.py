@@ -45,7 +45,8 @@ SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_creat
            "cq_plain_forward", "cq_batch_enable_timing", "cq_batch_pass_times",
            "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image",
            "cq_server_open", "cq_server_submit", "cq_server_advance", "cq_server_poll",
-           "cq_server_job_steps", "cq_server_result", "cq_server_release", "cq_server_close"]
+           "cq_server_job_steps", "cq_server_result", "cq_server_release", "cq_server_close",
+           "cq_forward_static"]
 
 _lib = None
 
@@ -135,6 +136,14 @@ class Context:
         d = make_desc(job, keep)
         rec = np.zeros(int(job["nt"]) * d.nr, np.float32)
         _chk(lib().cq_plain_forward(self.h, C.byref(d), rec.ctypes.data_as(_F)))
+        return rec.reshape(int(job["nt"]), d.nr)
+
+    def forward_static(self, job):
+        """cq_forward_static: the allocation-free variant of plain_forward."""
+        keep = []
+        d = make_desc(job, keep)
+        rec = np.zeros(int(job["nt"]) * d.nr, np.float32)
+        _chk(lib().cq_forward_static(self.h, C.byref(d), rec.ctypes.data_as(_F)))
         return rec.reshape(int(job["nt"]), d.nr)
 
     def batch(self, jobs):

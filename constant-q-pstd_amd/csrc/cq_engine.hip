@@ -75,10 +75,37 @@ __global__ __launch_bounds__(256) void k_copy_f4(const float4 *__restrict__ a, f
 }
 
 // ---------------------------------------------------------------------------
+// static scratch of the allocation-free entry point (cq_forward_static)
+// ---------------------------------------------------------------------------
+// Every GPU the library runs on holds one copy of this array until the
+// process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy (in host code the name is a host-side stand-in, not a device
+// address): cq_open looks up each device's own address once with
+// hipGetSymbolAddress and the entry point uses that.
+static __device__ __align__(256) char scratch[CQ_STATIC_BYTES];
+
+// Device address of `scratch` on the current device (looked up once per
+// device), or null with the error set.
+static char *scratch_address(int device)
+{
+    static std::mutex mu;
+    static std::map<int, char *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) return nullptr;
+    addr[device] = static_cast<char *>(ptr);
+    return static_cast<char *>(ptr);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 struct cq_ctx_s {
     int device = 0;
+    char *scratch = nullptr;   // this device's address of `scratch`
     hipStream_t stream = nullptr;
     std::map<std::vector<int>, float2 *> twiddles;  // radix sequence -> device twiddle table
     // open batches and servers; cq_close destroys what is left of them
@@ -121,14 +148,19 @@ static bool has_plan(int n, int need)
     return make_plan(n, need, p);
 }
 
-static int get_plan(cq_ctx ctx, int n, int need, CqPlan &p, const float2 *&tw)
+static int pool_take(cq_batch_s *b, size_t bytes, void **out);
+
+// Plan and twiddle table of an n-point transform for `need` points per
+// workgroup.  The tables are cached in the context, or, for a pooled batch,
+// placed in its pool.
+static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, CqPlan &p, const float2 *&tw)
 {
     if (!make_plan(n, need, p))
         return set_err(CQ_EUNSUPPORTED,
                        "FFT length %d: no factorisation into radices 2..17 for %d points per workgroup", n, need);
     std::vector<int> key(p.rad, p.rad + p.ns);
     auto itp = ctx->twiddles.find(key);
-    if (itp != ctx->twiddles.end()) {
+    if (!pb && itp != ctx->twiddles.end()) {
         tw = itp->second;
         return CQ_OK;
     }
@@ -146,6 +178,14 @@ static int get_plan(cq_ctx ctx, int n, int need, CqPlan &p, const float2 *&tw)
         o += (size_t)(R - 1) * Ns;
     }
     float2 *d = nullptr;
+    if (pb) {
+        const int rc = pool_take(pb, sizeof(float2) * n, (void **)&d);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        tw = d;
+        return CQ_OK;
+    }
     HIPCHK(hipMalloc(&d, sizeof(float2) * n));
     HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
     ctx->twiddles[key] = d;
@@ -345,12 +385,20 @@ struct cq_batch_s {
     hipStream_t side[kGroups] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kGroups] = {};
     bool split = false;
+    // A pooled batch (cq_forward_static) allocates nothing: its arenas,
+    // twiddle tables, job table and work items are carved out of `pool`
+    // (the device's static scratch), and it runs on the context's stream
+    // alone (no side streams or events).
+    char *pool = nullptr;
+    size_t pool_cap = 0, pool_off = 0;
     ~cq_batch_s()
     {
-        for (auto &j : jobs) (void)hipFree(j.mem);
-        (void)hipFree(djobs);
-        for (int r = 0; r < 2; r++)
-            for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
+        if (!pool) {
+            for (auto &j : jobs) (void)hipFree(j.mem);
+            (void)hipFree(djobs);
+            for (int r = 0; r < 2; r++)
+                for (int i = 0; i < P_COUNT; i++) (void)hipFree(ditems[r][i]);
+        }
         for (auto e : ev) (void)hipEventDestroy(e);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         for (int g = 1; g < kGroups; g++) {
@@ -359,6 +407,19 @@ struct cq_batch_s {
         }
     }
 };
+
+// `bytes` of a pooled batch's pool (256-byte aligned), or CQ_ENOMEM when
+// the static scratch is exhausted.
+static int pool_take(cq_batch_s *b, size_t bytes, void **out)
+{
+    const size_t o = (b->pool_off + 255) & ~(size_t)255;
+    if (o + bytes > b->pool_cap)
+        return set_err(CQ_ENOMEM, "the shot needs more than the %zu-byte static scratch of cq_forward_static",
+                       b->pool_cap);
+    *out = b->pool + o;
+    b->pool_off = o + bytes;
+    return CQ_OK;
+}
 
 extern "C" const char *cq_last_error(void) { return g_err.c_str(); }
 extern "C" const char *cq_version(void) { return CQ_VERSION; }
@@ -376,6 +437,11 @@ extern "C" int cq_open(int device, cq_ctx *out)
     if (e != hipSuccess) {
         delete c;
         return set_err(CQ_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->scratch = scratch_address(device);
+    if (!c->scratch) {
+        delete c;
+        return set_err(CQ_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d", device);
     }
     *out = c;
     g_err.clear();
@@ -479,8 +545,15 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     // cw/2 lines of nzz
     const int need_x = std::max((tz / 2) * nxx, q ? std::max((base.r1_fused ? 2 : 1) * tk, tk2) * nxx : 0),
               need_z = (cw / 2) * nzz;
-    if ((rc = get_plan(ctx, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
-    if ((rc = get_plan(ctx, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
+    if (bytes_only) {
+        // footprint only: the plans must exist, nothing is uploaded
+        if (!make_plan(nxx, need_x, base.px) || !make_plan(nzz, need_z, base.pz))
+            return set_err(CQ_EUNSUPPORTED, "window %dx%d: no FFT plan for the batch's tiles", nxx, nzz);
+    } else {
+        cq_batch_s *pb = b->pool ? b : nullptr;
+        if ((rc = get_plan(ctx, pb, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
+        if ((rc = get_plan(ctx, pb, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
+    }
 
     const StateLayout sl = state_layout(nxx, ldz, ldk, pml, q);
     Arena A;
@@ -530,9 +603,14 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         return CQ_OK;
     }
     char *mem = nullptr;
-    hipError_t e = hipMalloc(&mem, total);
-    if (e != hipSuccess) return set_err(CQ_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e));
-    h.mem = mem;
+    if (b->pool) {
+        rc = pool_take(b, total, (void **)&mem);
+        if (rc) return rc;
+    } else {
+        hipError_t e = hipMalloc(&mem, total);
+        if (e != hipSuccess) return set_err(CQ_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e));
+        h.mem = mem;
+    }
     h.bytes = total;
     h.nroles = nrole;
     h.state_bytes = sl.bytes;
@@ -837,11 +915,18 @@ static int rebuild_tables(cq_batch b)
     hipStream_t s = b->ctx->stream;
     const size_t n = b->jobs.size();
     if (n > b->djobs_cap || !b->djobs) {
-        (void)hipFree(b->djobs);
-        b->djobs = nullptr;
-        b->djobs_cap = std::max(n, (size_t)2 * b->djobs_cap);
-        if (b->djobs_cap == 0) b->djobs_cap = 1;
-        HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * 2 * b->djobs_cap));
+        if (b->pool) {
+            // a pooled batch builds its tables once
+            b->djobs_cap = std::max(n, (size_t)1);
+            const int rc = pool_take(b, sizeof(JobD) * 2 * b->djobs_cap, (void **)&b->djobs);
+            if (rc) return rc;
+        } else {
+            (void)hipFree(b->djobs);
+            b->djobs = nullptr;
+            b->djobs_cap = std::max(n, (size_t)2 * b->djobs_cap);
+            if (b->djobs_cap == 0) b->djobs_cap = 1;
+            HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * 2 * b->djobs_cap));
+        }
     }
     const int cap = (int)b->djobs_cap;
     std::vector<JobD> hd(2 * (size_t)cap);
@@ -889,10 +974,16 @@ static int rebuild_tables(cq_batch b)
             b->goff[r][p][kGroups] = (int)ord[r][p].size();
             if (ord[r][p].empty()) continue;
             if (ord[r][p].size() > b->ditems_cap[r][p]) {
-                (void)hipFree(b->ditems[r][p]);
-                b->ditems[r][p] = nullptr;
-                b->ditems_cap[r][p] = std::max(ord[r][p].size(), 2 * b->ditems_cap[r][p]);
-                HIPCHK(hipMalloc(&b->ditems[r][p], sizeof(int2) * b->ditems_cap[r][p]));
+                if (b->pool) {
+                    b->ditems_cap[r][p] = ord[r][p].size();
+                    const int rc = pool_take(b, sizeof(int2) * b->ditems_cap[r][p], (void **)&b->ditems[r][p]);
+                    if (rc) return rc;
+                } else {
+                    (void)hipFree(b->ditems[r][p]);
+                    b->ditems[r][p] = nullptr;
+                    b->ditems_cap[r][p] = std::max(ord[r][p].size(), 2 * b->ditems_cap[r][p]);
+                    HIPCHK(hipMalloc(&b->ditems[r][p], sizeof(int2) * b->ditems_cap[r][p]));
+                }
             }
             HIPCHK(hipMemcpyAsync(b->ditems[r][p], ord[r][p].data(), sizeof(int2) * ord[r][p].size(),
                                   hipMemcpyHostToDevice, s));
@@ -1073,7 +1164,7 @@ static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
 // bracket single passes.
 static int fork_side(cq_batch b)
 {
-    if (b->timing) return CQ_OK;
+    if (b->timing || b->pool) return CQ_OK;
     if (!b->fork_ev) {
         HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
         for (int g = 1; g < kGroups; g++) {
@@ -1730,5 +1821,43 @@ extern "C" int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec)
     }
     if (pinned) (void)hipHostFree(pinned);
     cq_batch_destroy(b);
+    return rc;
+}
+
+// The allocation-free variant of cq_plain_forward: the same passes with the
+// same tile shapes (so the same results, bit for bit), every device buffer
+// carved out of this device's static scratch.
+extern "C" int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec)
+{
+    if (!ctx || !job || !rec) return set_err(CQ_EINVAL, "cq_forward_static: bad arguments");
+    if (is_rtm(job->kind))
+        return set_err(CQ_EINVAL, "cq_forward_static: modeling shots only (kind %d is RTM)", job->kind);
+    int rc = validate(*job, 0);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->scratch) return set_err(CQ_EDEVICE, "cq_forward_static: no static scratch on device %d", ctx->device);
+    cq_batch_s b;
+    b.ctx = ctx;
+    rc = choose_tiles(&b, {TileReq{job->nxx, job->nzz, is_q(job->kind)}});
+    if (rc) return rc;
+    // the arena's size is known before anything is placed: refuse a shot
+    // that cannot fit, with its size
+    JobH probe;
+    size_t need = 0;
+    rc = build_job(&b, *job, probe, &need);
+    if (rc) return rc;
+    if (need > (size_t)CQ_STATIC_BYTES / 10 * 9)
+        return set_err(CQ_EINVAL, "cq_forward_static: the shot needs %zu device bytes; the static scratch holds %zu",
+                       need, (size_t)CQ_STATIC_BYTES);
+    b.pool = ctx->scratch;
+    b.pool_cap = CQ_STATIC_BYTES;
+    // the previous call's contents are still in the scratch: build_job clears
+    // the arena (state, checkpoints, gathers) on the stream before any pass
+    b.jobs.resize(1);
+    rc = build_job(&b, *job, b.jobs[0]);
+    if (rc == CQ_OK) rc = rebuild_tables(&b);
+    if (rc == CQ_OK) rc = advance_to(&b, b.tmax);
+    if (rc == CQ_OK) rc = read_rec(ctx, b.jobs[0], rec);
+    (void)hipStreamSynchronize(ctx->stream);
     return rc;
 }

@@ -161,6 +161,28 @@ int cq_server_close(cq_server s);
  * Stops with CQ_ENUMERIC at the first gather row holding a non-finite value. */
 int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec);
 
+/* Size of the static device scratch of cq_forward_static (64 MiB per GPU,
+ * held from the library's load until the process ends). */
+#define CQ_STATIC_BYTES (64LL << 20)
+
+/* Allocation-free variant of cq_plain_forward.  No counterpart in the
+ * reference (whose acforward allocates per call, sgpstd-qmodeling/acps4.cu:797-928):
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * one modeling shot (kind ACOUSTIC or VISCO) to its end and copies the gather
+ * (nt*nr floats) to rec; the results equal cq_plain_forward's bit for bit
+ * (the same passes and tile shapes).  Every device buffer -- wavefield
+ * state, coefficients, FFT twiddles, gather, work lists -- lives in one
+ * static device array of CQ_STATIC_BYTES per GPU, and the call makes no
+ * hipMalloc, hipFree, hipHostMalloc or stream creation; it synchronises the
+ * context's stream before returning.  A shot whose device footprint exceeds
+ * 90% of CQ_STATIC_BYTES (about a 1200 x 680 acoustic window at nt = 3000)
+ * is refused with CQ_EINVAL.  The array is shared by every call on that
+ * GPU and keeps the last call's contents (each call clears what must start
+ * at zero): calls that overlap on one GPU, from several threads or
+ * contexts, race and must be serialised by the caller. */
+int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec);
+
 /* Per-pass device timing with HIP events on the engine's stream.  While
  * enabled, every launch of cq_batch_step is bracketed by events; passes are
  * numbered 0..7 (cq_pass_name gives the kernel name): acoustic row, acoustic
