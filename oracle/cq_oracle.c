@@ -429,11 +429,11 @@ static void model_from_args(orc_model *m, int nxx, int nzz, real dt,
  *   p_out     : optional final p (nxx*nzz)
  * Returns 0.
  */
-int orc_forward(int kind, int nxx, int nzz, int nt, real dt,
-                const real *const *fields, const real *const *c1d, const real *const *k2,
-                const real *rik, int xsg, int zsg,
-                int nr, const int *gxg, const int *gzg, real *rec, real *p_out,
-                real *vx_out, real *vz_out)
+int orc_forward_ic(int kind, int nxx, int nzz, int nt, real dt,
+                   const real *const *fields, const real *const *c1d, const real *const *k2,
+                   const real *rik, int xsg, int zsg,
+                   int nr, const int *gxg, const int *gzg, real *rec, real *p_out,
+                   real *vx_out, real *vz_out, const real *p0, const real *vx0, const real *vz0)
 {
     orc_model m;
     orc_state s;
@@ -442,6 +442,11 @@ int orc_forward(int kind, int nxx, int nzz, int nt, real dt,
     model_from_args(&m, nxx, nzz, dt, fields, c1d, k2);
     state_alloc(&s, N);
     work_init(&w, nxx, nzz);
+    /* optional initial wavefield (the reference always starts from rest,
+     * initialize_kernel; an initial state only serves the closed-form tests) */
+    if (p0) memcpy(s.p, p0, sizeof(real) * N);
+    if (vx0) memcpy(s.vx, vx0, sizeof(real) * N);
+    if (vz0) memcpy(s.vz, vz0, sizeof(real) * N);
     for (int t = 0; t < nt; t++) {
         step_div(&m, &s, &w);
         if (kind == 0) {
@@ -464,6 +469,16 @@ int orc_forward(int kind, int nxx, int nzz, int nt, real dt,
     state_free(&s);
     work_free(&w);
     return 0;
+}
+
+int orc_forward(int kind, int nxx, int nzz, int nt, real dt,
+                const real *const *fields, const real *const *c1d, const real *const *k2,
+                const real *rik, int xsg, int zsg,
+                int nr, const int *gxg, const int *gzg, real *rec, real *p_out,
+                real *vx_out, real *vz_out)
+{
+    return orc_forward_ic(kind, nxx, nzz, nt, dt, fields, c1d, k2, rik, xsg, zsg, nr, gxg, gzg, rec, p_out,
+                          vx_out, vz_out, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

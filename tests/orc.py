@@ -85,10 +85,11 @@ class _Args:
         self.gzg = np.ascontiguousarray(job["gzg"], np.int32)
 
 
-def forward(job, bits=64, want_state=False):
+def forward(job, bits=64, want_state=False, init=None):
     """One-shot forward modeling on the CPU.  job: dict with kind (0 acoustic,
     1 viscoacoustic), nxx, nzz, nt, dt and the float32 arrays of the job
-    (window fields, 1-D coefficients, k powers, rik, gxg, gzg, xsg, zsg)."""
+    (window fields, 1-D coefficients, k powers, rik, gxg, gzg, xsg, zsg).
+    init: optional {"p", "vx", "vz"} initial wavefield (nxx*nzz each)."""
     L = lib(bits)
     dt = _dt(bits)
     a = _Args(job, bits)
@@ -99,12 +100,21 @@ def forward(job, bits=64, want_state=False):
     vx = np.zeros(N, dt)
     vz = np.zeros(N, dt)
     real = C.c_double if bits == 64 else C.c_float
-    L.orc_forward(C.c_int(int(job["kind"])), C.c_int(int(job["nxx"])), C.c_int(int(job["nzz"])),
-                  C.c_int(nt), real(float(job["dt"])), a.fields, a.c1d, a.k2, _ptr(a.rik, bits),
-                  C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])), C.c_int(nr),
-                  a.gxg.ctypes.data_as(C.POINTER(C.c_int)),
-                  a.gzg.ctypes.data_as(C.POINTER(C.c_int)), _ptr(rec, bits), _ptr(p, bits),
-                  _ptr(vx, bits), _ptr(vz, bits))
+    ic = []
+    for k in ("p", "vx", "vz"):
+        if init is not None and init.get(k) is not None:
+            v = np.ascontiguousarray(init[k], dt).ravel()
+            assert v.size == N
+            ic.append(v)
+        else:
+            ic.append(None)
+    icp = [None if v is None else _ptr(v, bits) for v in ic]
+    L.orc_forward_ic(C.c_int(int(job["kind"])), C.c_int(int(job["nxx"])), C.c_int(int(job["nzz"])),
+                     C.c_int(nt), real(float(job["dt"])), a.fields, a.c1d, a.k2, _ptr(a.rik, bits),
+                     C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])), C.c_int(nr),
+                     a.gxg.ctypes.data_as(C.POINTER(C.c_int)),
+                     a.gzg.ctypes.data_as(C.POINTER(C.c_int)), _ptr(rec, bits), _ptr(p, bits),
+                     _ptr(vx, bits), _ptr(vz, bits), icp[0], icp[1], icp[2])
     rec = rec.reshape(nt, nr)
     if want_state:
         return rec, {"p": p, "vx": vx, "vz": vz}
