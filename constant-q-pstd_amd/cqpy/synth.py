@@ -199,14 +199,15 @@ def make_window_job(kind, vpt, rhot, nxt, nzt, dx, dz, dt, nt, fd, left_col, nx,
     return job
 
 
-def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None):
+def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, widen=True):
     """Batched multi-shot workload: 128 shots on a 600 x 2400 model, 40-point
     CPML, dx = dz = 5 m, dt = 0.4 ms, layered vp 1500-4500 m/s with a salt-like
     ellipse at 4500 m/s, Q 30-300, 20 Hz Ricker, shots every 18 cells at the
     surface, receivers on every surface column inside each shot's window.
     Windows are cropped to +-600 cells around the source (uneven near the
-    model edges) and then widened to the right to the next FFT-friendly
-    padded width.  kinds[i] = 0 (acoustic) or 1 (viscoacoustic)."""
+    model edges; interior shots 1281 x 680 padded, 1281 = 3 7 61) and, with
+    widen, widened to the right to the next 7-smooth padded width (1296).
+    kinds[i] = 0 (acoustic) or 1 (viscoacoustic)."""
     nxt, nzt, dx, pml = 2400, 600, 5.0, 40
     vp, rho, q = layered_model(nxt, nzt, dx, seed=301, nlayers=6, vmin=1500.0, vmax=4500.0,
                                qrange=(30.0, 300.0), salt=(1200, 330, 260, 90, 4500.0))
@@ -217,7 +218,8 @@ def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None):
         left = max(0, c - 600)
         right = min(nxt - 1, c + 600)
         nx = right - left + 1
-        nx = fft_friendly(nx + 2 * pml) - 2 * pml
+        if widen:
+            nx = fft_friendly(nx + 2 * pml) - 2 * pml
         kind = 0 if kinds is None else kinds[i]
         jobs.append(make_window_job(kind, vp, rho, nxt, nzt, dx, dx, 0.0004, nt, 20.0, left, nx, c,
                                     2, np.arange(left, right + 1), 2, pml, qt=q, fo=20.0,
@@ -268,11 +270,12 @@ def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 1216), nz_range
     return out
 
 
-def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402):
+def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402, widen=True):
     """Q-RTM workload: shots over an 800 x 3000 model with a 40-point CPML,
     dx = dz = 6.25 m, dt = 0.5 ms, 20 Hz Ricker, receivers +-1000 cells
-    around each source at the surface (window widened to 2020 columns so the
-    padded width 2100 = 2^2 3 5^2 7 is FFT-friendly), `snaps` checkpoints,
+    around each source at the surface (2001 columns, padded 2081, a prime;
+    with widen the window is widened to 2020 columns so that the padded width
+    2100 = 2^2 3 5^2 7 is 7-smooth), `snaps` checkpoints,
     stabiliser 1e-20 on both wavefields.  The migration model is a layered
     vp 1500-4500 m/s / Q 25-250 model smoothed with a Gaussian of 12 cells;
     observed data are seeded random traces (the run time does not depend on
@@ -280,7 +283,7 @@ def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402):
     nxt, nzt, dx, pml = 3000, 800, 6.25, 40
     vp, rho, q = layered_model(nxt, nzt, dx, seed=seed, nlayers=8, vmin=1500.0, vmax=4500.0,
                                qrange=(25.0, 250.0), smooth_sigma=12.0)
-    nx = fft_friendly(2001 + 2 * pml) - 2 * pml
+    nx = fft_friendly(2001 + 2 * pml) - 2 * pml if widen else 2001
     first, last = 1000, nxt - 1001
     rng = np.random.default_rng(obs_seed)
     jobs = []

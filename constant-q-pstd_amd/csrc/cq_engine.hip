@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -108,12 +109,18 @@ struct cq_ctx_s {
     char *scratch = nullptr;   // this device's address of `scratch`
     hipStream_t stream = nullptr;
     std::map<std::vector<int>, float2 *> twiddles;  // radix sequence -> device twiddle table
+    // (n, M) -> device chirp and spectrum tables of an n-point Bluestein transform
+    std::map<std::pair<int, int>, std::pair<float2 *, float2 *>> bluestein;
     // open batches and servers; cq_close destroys what is left of them
     std::set<cq_batch_s *> batches;
     std::set<cq_server_s *> servers;
     ~cq_ctx_s()
     {
         for (auto &kv : twiddles) (void)hipFree(kv.second);
+        for (auto &kv : bluestein) {
+            (void)hipFree(kv.second.first);
+            (void)hipFree(kv.second.second);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -123,7 +130,7 @@ struct cq_ctx_s {
 // and powers of three as 9/3, then 5, 7, 11, 13, 17, but only radices whose
 // per-stage capacity (cq_stage_capacity) holds `need` points.  False if n
 // has another prime factor or a prime factor's radix is too small.
-static bool make_plan(int n, int need, CqPlan &p)
+static bool make_direct_plan(int n, int need, CqPlan &p)
 {
     memset(&p, 0, sizeof p);
     p.n = n;
@@ -142,22 +149,134 @@ static bool make_plan(int n, int need, CqPlan &p)
     return m == 1;
 }
 
+// True if every prime factor of n is in `primes`.
+static bool smooth(int n, std::initializer_list<int> primes)
+{
+    if (n < 1) return false;
+    for (int q : primes)
+        while (n % q == 0) n /= q;
+    return n == 1;
+}
+
+// Length of the inner transforms of a Bluestein transform of n points: the
+// smallest 7-smooth M >= 2 n - 1 (the cyclic convolution then equals the
+// linear one of two n-point sequences).
+static int bluestein_len(int n)
+{
+    for (int m = 2 * n - 1;; m++)
+        if (smooth(m, {2, 3, 5, 7})) return m;
+}
+
+// Points per line a transform of n points occupies: n when its prime factors
+// are radices of the stages (2, 3, 5, 7, 11, 13, 17), else the Bluestein
+// length.
+static int fft_len(int n) { return smooth(n, {2, 3, 5, 7, 11, 13, 17}) ? n : bluestein_len(n); }
+
+// Plan of an n-point transform for `need` points (need / n lines) per
+// workgroup: direct, or Bluestein over fft_len(n) points per line.
+static bool make_plan(int n, int need, CqPlan &p)
+{
+    const int M = fft_len(n);
+    if (M == n) return make_direct_plan(n, need, p);
+    const int lines = (need + n - 1) / n;
+    if (!make_direct_plan(M, lines * M, p)) return false;
+    p.bn = n;
+    return true;
+}
+
 static bool has_plan(int n, int need)
 {
     CqPlan p;
     return make_plan(n, need, p);
 }
 
+// In-place FFT of a 7-smooth length in double precision (host), by
+// decimation in time over the smallest prime factor; sign -1: exp(-i).
+static void host_fft(std::vector<std::complex<double>> &x, int sign)
+{
+    const int n = (int)x.size();
+    if (n == 1) return;
+    int p = 2;
+    while (n % p) p++;
+    const int m = n / p;
+    std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(m));
+    for (int r = 0; r < p; r++)
+        for (int j = 0; j < m; j++) sub[r][j] = x[(size_t)j * p + r];
+    for (int r = 0; r < p; r++) host_fft(sub[r], sign);
+    for (int k = 0; k < n; k++) {
+        std::complex<double> acc = 0;
+        for (int r = 0; r < p; r++)
+            acc += sub[r][k % m] * std::polar(1.0, sign * 2.0 * M_PI * (double)((long long)r * k % n) / n);
+        x[k] = acc;
+    }
+}
+
+// Bluestein tables of an n-point transform with M-point inner transforms:
+// chirp c[j] = exp(-i pi j^2 / n) (j^2 reduced mod 2n in integers), and
+// B = FFT_M(b) / M of b[m] = conj c[|m|] for |m| < n, wrapped cyclically.
+static void bluestein_tables(int n, int M, std::vector<float2> &c, std::vector<float2> &B)
+{
+    std::vector<std::complex<double>> cd(n), b(M, 0.0);
+    c.resize(n);
+    for (int j = 0; j < n; j++) {
+        const long long r = (long long)j * j % (2LL * n);
+        cd[j] = std::polar(1.0, -M_PI * (double)r / n);
+        c[j] = make_float2((float)cd[j].real(), (float)cd[j].imag());
+    }
+    b[0] = std::conj(cd[0]);
+    for (int m = 1; m < n; m++) b[m] = b[M - m] = std::conj(cd[m]);
+    host_fft(b, -1);
+    B.resize(M);
+    for (int k = 0; k < M; k++) B[k] = make_float2((float)(b[k].real() / M), (float)(b[k].imag() / M));
+}
+
 static int pool_take(cq_batch_s *b, size_t bytes, void **out);
 
+// Uploads `count` float2 host values: into the pool of a pooled batch, else
+// into a new device buffer that the context caches.
+static int upload_table(cq_ctx ctx, cq_batch_s *pb, const std::vector<float2> &h, float2 **out)
+{
+    const size_t bytes = sizeof(float2) * h.size();
+    if (pb) {
+        const int rc = pool_take(pb, bytes, (void **)out);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(*out, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return CQ_OK;
+    }
+    HIPCHK(hipMalloc(out, bytes));
+    HIPCHK(hipMemcpy(*out, h.data(), bytes, hipMemcpyHostToDevice));
+    return CQ_OK;
+}
+
 // Plan and twiddle table of an n-point transform for `need` points per
-// workgroup.  The tables are cached in the context, or, for a pooled batch,
-// placed in its pool.
+// workgroup (and a Bluestein plan's chirp tables).  The tables are cached in
+// the context, or, for a pooled batch, placed in its pool.
 static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, CqPlan &p, const float2 *&tw)
 {
     if (!make_plan(n, need, p))
         return set_err(CQ_EUNSUPPORTED,
-                       "FFT length %d: no factorisation into radices 2..17 for %d points per workgroup", n, need);
+                       "FFT length %d (transform length %d): no factorisation into radices 2..17 for %d points "
+                       "per workgroup", n, fft_len(n), need / n * fft_len(n));
+    if (p.bn) {
+        const std::pair<int, int> bk(p.bn, p.n);
+        auto itb = ctx->bluestein.find(bk);
+        if (!pb && itb != ctx->bluestein.end()) {
+            p.chirp = itb->second.first;
+            p.bspec = itb->second.second;
+        } else {
+            std::vector<float2> c, B;
+            bluestein_tables(p.bn, p.n, c, B);
+            float2 *dc = nullptr, *dB = nullptr;
+            int rc = upload_table(ctx, pb, c, &dc);
+            if (rc == CQ_OK) rc = upload_table(ctx, pb, B, &dB);
+            if (rc) return rc;
+            if (!pb) ctx->bluestein[bk] = std::make_pair(dc, dB);
+            p.chirp = dc;
+            p.bspec = dB;
+        }
+    }
+    const int N = p.n;   // points of the stages' transforms (M for Bluestein)
     std::vector<int> key(p.rad, p.rad + p.ns);
     auto itp = ctx->twiddles.find(key);
     if (!pb && itp != ctx->twiddles.end()) {
@@ -165,8 +284,8 @@ static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, CqPlan &p, cons
         return CQ_OK;
     }
     // per-stage tables (cq_fft.h): stage s holds w_{Ns R}^(r k) at
-    // (r-1) Ns + k, r = 1..R-1, k < Ns; n - 1 entries in all (one spare)
-    std::vector<float2> h(n, make_float2(1.f, 0.f));
+    // (r-1) Ns + k, r = 1..R-1, k < Ns; N - 1 entries in all (one spare)
+    std::vector<float2> h(N, make_float2(1.f, 0.f));
     size_t o = 0;
     for (int s = 0, Ns = 1; s < p.ns; Ns *= p.rad[s], s++) {
         const int R = p.rad[s];
@@ -178,17 +297,9 @@ static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, CqPlan &p, cons
         o += (size_t)(R - 1) * Ns;
     }
     float2 *d = nullptr;
-    if (pb) {
-        const int rc = pool_take(pb, sizeof(float2) * n, (void **)&d);
-        if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        tw = d;
-        return CQ_OK;
-    }
-    HIPCHK(hipMalloc(&d, sizeof(float2) * n));
-    HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    ctx->twiddles[key] = d;
+    const int rc = upload_table(ctx, pb, h, &d);
+    if (rc) return rc;
+    if (!pb) ctx->twiddles[key] = d;
     tw = d;
     return CQ_OK;
 }
@@ -349,6 +460,14 @@ struct JobH {
 };
 
 // One launch of the per-step sequence.
+static const size_t kLdsMax = 160 * 1024;
+// LDS budget per workgroup that the tile choice aims for: 80 KB lets two
+// workgroups share a CU; tiles fall back to kLdsMax when a window needs it.
+#ifndef CQ_LDS_TILE
+#define CQ_LDS_TILE (80 * 1024)
+#endif
+static const size_t kLdsTile = CQ_LDS_TILE;
+
 enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_QC2, P_QR2, P_QC3, P_QR3, P_COUNT };
 static const int kGroups = 4;   // launch groups of a batch (see cq_batch_s::goff)
 static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1",
@@ -507,6 +626,7 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
 }
 
 static bool r1_fused_for(int nxx, int tk);
+static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out);
 
 // Allocates and uploads one shot for the batch's tile shapes.  With
 // bytes_only != null only its device footprint is computed.
@@ -553,6 +673,23 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         cq_batch_s *pb = b->pool ? b : nullptr;
         if ((rc = get_plan(ctx, pb, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
         if ((rc = get_plan(ctx, pb, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
+    }
+    // lines hold the transform length (M points for a Bluestein plan)
+    base.twlx = base.px.n;
+    base.twlz = base.pz.n;
+    base.lsx = cq_line_stride(base.px.n);
+    base.lsz = cq_line_stride(base.pz.n);
+    {
+        // the passes' LDS for this window with the batch's tiles; a
+        // Bluestein length may need more than the batch's widest window
+        size_t need_lds[P_COUNT];
+        job_lds(b, nxx, nzz, q, need_lds);
+        for (int pp = 0; pp < P_COUNT; pp++)
+            if (need_lds[pp] > kLdsMax)
+                return set_err(CQ_EUNSUPPORTED, "window %dx%d: pass %s needs %zu bytes of LDS (at most %zu)", nxx, nzz,
+                               kPassName[pp], need_lds[pp], kLdsMax);
+        if (!bytes_only)
+            for (int pp = 0; pp < P_COUNT; pp++) b->lds[pp] = std::max(b->lds[pp], need_lds[pp]);
     }
 
     const StateLayout sl = state_layout(nxx, ldz, ldk, pml, q);
@@ -795,17 +932,15 @@ static std::vector<int2> xcd_order(const std::vector<int2> &items)
     return out;
 }
 
-static const size_t kLdsMax = 160 * 1024;
-// LDS budget per workgroup that the tile choice aims for: 80 KB lets two
-// workgroups share a CU; tiles fall back to kLdsMax when a window needs it.
-#ifndef CQ_LDS_TILE
-#define CQ_LDS_TILE (80 * 1024)
-#endif
-static const size_t kLdsTile = CQ_LDS_TILE;
 
 // LDS bytes of a workgroup holding the twiddle table of an n-point transform
-// and `lines` lines of it (cq_line_stride slots apart).
-static size_t lds_lines(int n, int lines) { return (size_t)(n + lines * cq_line_stride(n)) * 8; }
+// and `lines` lines of it (cq_line_stride slots apart), for the transform
+// length fft_len(n) (M points for a Bluestein length).
+static size_t lds_lines(int n, int lines)
+{
+    const int m = fft_len(n);
+    return (size_t)(m + lines * cq_line_stride(m)) * 8;
+}
 // Candidate tile widths, largest first: z rows per row tile (two per packed
 // line), columns per column tile (two per packed line), kz rows per kz-row
 // tile.  Widths that are not powers of two fill a workgroup's 512 threads
@@ -826,6 +961,21 @@ static bool r1_ok(int nxx, int tk, size_t lim)
 {
     return (has_plan(nxx, 2 * tk * nxx) && lds_lines(nxx, 3 * tk) <= lim) ||
            (has_plan(nxx, tk * nxx) && lds_lines(nxx, 2 * tk) <= lim);
+}
+
+// LDS of each pass for one window of the given kind with the batch's tiles.
+static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out)
+{
+    for (int p = 0; p < P_COUNT; p++) out[p] = 0;
+    if (!q) {
+        out[P_AROW] = lds_lines(nxx, b->tz / 2);
+        out[P_ACOL] = lds_lines(nzz, b->cw / 2);
+    } else {
+        out[P_QC1] = out[P_QC2] = out[P_QC3] = lds_lines(nzz, b->cwq / 2);
+        out[P_QR1] = r1_lds(nxx, b->tk);
+        out[P_QR2] = lds_lines(nxx, b->tk2);
+        out[P_QR3] = lds_lines(nxx, b->tz / 2);
+    }
 }
 
 // LDS of each pass for windows up to ax x az (acoustic) and qx x qz
@@ -860,10 +1010,12 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
     for (size_t i = 0; i < req.size(); i++) {
         const TileReq &j = req[i];
         if (j.q) qxs.push_back(j.nxx);
-        if (!has_plan(j.nxx, 1))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nxx=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", (int)i, j.nxx);
-        if (!has_plan(j.nzz, 1))
-            return set_err(CQ_EUNSUPPORTED, "job %d: FFT length nzz=%d has a prime factor other than 2, 3, 5, 7, 11, 13, 17", (int)i, j.nzz);
+        if (!has_plan(j.nxx, j.nxx))
+            return set_err(CQ_EUNSUPPORTED, "job %d: no FFT plan for nxx=%d (transform length %d)", (int)i, j.nxx,
+                           fft_len(j.nxx));
+        if (!has_plan(j.nzz, j.nzz))
+            return set_err(CQ_EUNSUPPORTED, "job %d: no FFT plan for nzz=%d (transform length %d)", (int)i, j.nzz,
+                           fft_len(j.nzz));
         const bool q = j.q;
         auto row_bad = [&](int t, size_t lim) {
             return !has_plan(j.nxx, (t / 2) * j.nxx) || lds_lines(j.nxx, t / 2) > lim;

@@ -11,6 +11,16 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Global address space qualifier for device pointers (global_load/store
+// instead of flat accesses).  It only exists in the device compilation pass;
+// the host pass, which never dereferences these pointers, sees plain ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CQ_GLOBAL __attribute__((address_space(1)))
+#else
+#define CQ_GLOBAL
+#endif
+
+
 // Workgroups of 512 threads.  A thread holds the butterflies it computes in
 // one stage in registers (cq_nb(R) of them: 16-18 complex values, 32 for
 // radices 7/8 whose butterflies need few temporaries), which keeps every pass
@@ -29,10 +39,18 @@ __host__ __device__ constexpr int cq_nb(int R)
     return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : 1;
 }
 
+// A transform plan: stages of radices rad[0 .. ns) for n points.  A length
+// with a prime factor above 17 is done by Bluestein's chirp-z algorithm:
+// then n is the smooth length M >= 2 bn - 1 of the inner transforms, bn the
+// true length, chirp[j] = exp(-i pi j^2 / bn) (j < bn) and bspec the M-point
+// spectrum of the conjugate chirp, wrapped cyclically, divided by M.
 struct CqPlan {
     int n;
     int ns;
     int rad[CQ_MAXST];
+    int bn;   // 0: direct transform of n points
+    const float2 *chirp;   // device pointers (global memory)
+    const float2 *bspec;
 };
 
 // LDS line layout: a line holds its n values in order (slot i = value i) and
@@ -225,15 +243,6 @@ template <bool INV> struct Dft<9, INV> : DftOdd<9, INV> {};
 template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
 template <bool INV> struct Dft<17, INV> : DftOdd<17, INV> {};
-
-// Global address space qualifier for device pointers (global_load/store
-// instead of flat accesses).  It only exists in the device compilation pass;
-// the host pass, which never dereferences these pointers, sees plain ones.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define CQ_GLOBAL __attribute__((address_space(1)))
-#else
-#define CQ_GLOBAL
-#endif
 
 // Dynamic LDS of every kernel.  The out-of-line stage functions address it
 // through this symbol (by float2 offsets) so that their accesses compile to
@@ -429,36 +438,110 @@ __device__ __forceinline__ bool cq_fft_fixed(const CqPlan &p, int LS, int bo, in
            cq_fft_plan<INV, 2100, 1, 4, 3, 5, 5, 7>(p, LS, bo, L, to, mh);
 }
 
-__device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,
-                                           const float2 *tw, const float2 CQ_GLOBAL *__restrict__ mh)
+// The stages of plan p as run-time stages (any plan), multiplier on the last.
+template <bool INV>
+__device__ __forceinline__ void cq_fft_generic(const CqPlan &p, int bo, int LS, int L, int to,
+                                               const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    const int bo = (int)(buf - cq_lds);
-    int to = (int)(tw - cq_lds);
-    if (cq_fft_fixed<false>(p, LS, bo, L, to, mh)) return;
     int Ns = 1, mode = cq_first_mode(p);
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<false>(R, mode, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
+        cq_stage_rt<INV>(R, mode, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
         to += (R - 1) * Ns;
         Ns *= R;
         mode = mode == 1 ? 2 : 0;
     }
 }
 
+// Bluestein, first step: a[j] = x[j] chirp[j] (the forward transform of
+// conj x for an inverse one) for j < n, zero up to M, in place in each line.
+template <bool INV>
+__device__ __noinline__ void cq_chirp_in(int bo, int LS, int n, int M, int L, const float2 CQ_GLOBAL *__restrict__ c)
+{
+    bo = __builtin_amdgcn_readfirstlane(bo);
+    LS = __builtin_amdgcn_readfirstlane(LS);
+    n = __builtin_amdgcn_readfirstlane(n);
+    M = __builtin_amdgcn_readfirstlane(M);
+    float2 *const buf = cq_lds + bo;
+    // exact quotient by the hardware reciprocal, as in cq_stage (e < 2^15)
+    const float inv_m = __builtin_amdgcn_rcpf((float)M);
+    for (int e = threadIdx.x; e < L * M; e += CQ_NT) {
+        const int l = (int)(((float)e + 0.5f) * inv_m), j = e - l * M;
+        float2 *line = buf + l * LS;
+        float2 v = make_float2(0.f, 0.f);
+        if (j < n) {
+            float2 x = line[j];
+            if (INV) x.y = -x.y;
+            v = cmul(x, c[j]);
+        }
+        line[j] = v;
+    }
+    __syncthreads();
+}
+
+// Bluestein, last step: X[k] = chirp[k] r[k] (conjugated for an inverse
+// transform), times the optional multiplier, for k < n.
+template <bool INV>
+__device__ __noinline__ void cq_chirp_out(int bo, int LS, int n, int L, const float2 CQ_GLOBAL *__restrict__ c,
+                                          const float2 CQ_GLOBAL *__restrict__ mh)
+{
+    bo = __builtin_amdgcn_readfirstlane(bo);
+    LS = __builtin_amdgcn_readfirstlane(LS);
+    n = __builtin_amdgcn_readfirstlane(n);
+    float2 *const buf = cq_lds + bo;
+    const float inv_n = __builtin_amdgcn_rcpf((float)n);
+    for (int e = threadIdx.x; e < L * n; e += CQ_NT) {
+        const int l = (int)(((float)e + 0.5f) * inv_n), k = e - l * n;
+        float2 *line = buf + l * LS;
+        float2 y = cmul(line[k], c[k]);
+        if (INV) y.y = -y.y;
+        if (mh != nullptr) y = cmul(y, mh[k]);
+        line[k] = y;
+    }
+    __syncthreads();
+}
+
+// Transform of length p.bn by Bluestein's algorithm: with c[j] = e^{-i pi j^2/n},
+// X[k] = c[k] sum_j (x[j] c[j]) conj c[k - j], a linear convolution done as a
+// cyclic one of M >= 2n - 1 points (forward M-point transform, times bspec,
+// inverse); an inverse transform is the conjugate of the forward transform
+// of the conjugate.  The lines hold M points (LS >= the M-point line
+// stride); the n results replace the first n values.
+template <bool INV>
+__device__ __forceinline__ void cq_bluestein(const CqPlan &p, int bo, int LS, int L, int to,
+                                             const float2 CQ_GLOBAL *__restrict__ mh)
+{
+    const float2 CQ_GLOBAL *c = (const float2 CQ_GLOBAL *)p.chirp;
+    cq_chirp_in<INV>(bo, LS, p.bn, p.n, L, c);
+    cq_fft_generic<false>(p, bo, LS, L, to, (const float2 CQ_GLOBAL *)p.bspec);
+    cq_fft_generic<true>(p, bo, LS, L, to, nullptr);
+    cq_chirp_out<INV>(bo, LS, p.bn, L, c, mh);
+}
+
+__device__ __forceinline__ void cq_fft_fwd(const CqPlan &p, float2 *buf, int LS, int L,
+                                           const float2 *tw, const float2 CQ_GLOBAL *__restrict__ mh)
+{
+    const int bo = (int)(buf - cq_lds);
+    const int to = (int)(tw - cq_lds);
+    if (p.bn) {
+        cq_bluestein<false>(p, bo, LS, L, to, mh);
+        return;
+    }
+    if (cq_fft_fixed<false>(p, LS, bo, L, to, mh)) return;
+    cq_fft_generic<false>(p, bo, LS, L, to, mh);
+}
+
 __device__ __forceinline__ void cq_fft_inv(const CqPlan &p, float2 *buf, int LS, int L,
                                            const float2 *tw)
 {
     const int bo = (int)(buf - cq_lds);
-    int to = (int)(tw - cq_lds);
-    if (cq_fft_fixed<true>(p, LS, bo, L, to, nullptr)) return;
-    int Ns = 1, mode = cq_first_mode(p);
-    for (int s = 0; s < p.ns; s++) {
-        const int R = p.rad[s];
-        cq_stage_rt<true>(R, mode, bo, LS, p.n, L, Ns, to, nullptr);
-        to += (R - 1) * Ns;
-        Ns *= R;
-        mode = mode == 1 ? 2 : 0;
+    const int to = (int)(tw - cq_lds);
+    if (p.bn) {
+        cq_bluestein<true>(p, bo, LS, L, to, nullptr);
+        return;
     }
+    if (cq_fft_fixed<true>(p, LS, bo, L, to, nullptr)) return;
+    cq_fft_generic<true>(p, bo, LS, L, to, nullptr);
 }
 
 // Largest number of complex points one workgroup can transform per stage for

@@ -43,6 +43,9 @@ __host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
     CqPlan px, pz;                                                                                \
     const float2 G *twx;                                                                          \
     const float2 G *twz;                                                                          \
+    /* LDS geometry of the transforms: twiddle entries (the transform length, M for a */          \
+    /* Bluestein plan) and line strides along x and z */                                          \
+    int twlx, twlz, lsx, lsz;                                                                     \
     /* Hermitian-symmetrised 1-D multipliers / n: v->p (m), p->v (p), d2 (2) */                   \
     const float2 G *mxm; const float2 G *mxp; const float2 G *mx2;                                \
     const float2 G *mzm; const float2 G *mzp; const float2 G *mz2;                                \

@@ -98,9 +98,8 @@ __device__ __forceinline__ void load_tw(float2 *tw, const float2 CQ_GLOBAL *__re
 // ===========================================================================
 // CACHED: the pass reads src again later (ordinary loads, see ld4c)
 template <int Q, bool CACHED = false>
-__device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GLOBAL *__restrict__ src, int z0)
+__device__ __forceinline__ void row_fill(float2 *buf, int nxx, int LS, const float CQ_GLOBAL *__restrict__ src, int z0)
 {
-    const int LS = cq_line_stride(nxx);
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     cq_loop<8>(
         nxx * Q,
@@ -120,7 +119,7 @@ __device__ __forceinline__ void row_fill(float2 *buf, int nxx, const float CQ_GL
 template <int Q>
 __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 {
-    const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, pml = J.pml, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     const size_t g0 = (size_t)(z0 >> 2) * nxx * 4;
     cq_loop<4>(
@@ -169,7 +168,7 @@ __device__ __forceinline__ void row_vx_phase(const JobG &J, float2 *buf, int z0)
 template <int Q, bool TO_DIV = false>
 __device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, int z0)
 {
-    const int nxx = J.nxx, pml = J.pml, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, pml = J.pml, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     const size_t g0 = (size_t)(z0 >> 2) * nxx * 4;
     cq_loop<4>(
@@ -212,9 +211,9 @@ __device__ __forceinline__ void row_dvx_phase(const JobG &J, const float2 *buf, 
 // ===========================================================================
 // column helpers (real columns, packed; bufs = float view of the lines)
 // ===========================================================================
-__device__ __forceinline__ float *col_line(float *bufs, int nzz, int c)
+__device__ __forceinline__ float *col_line(float *bufs, int LS, int c)
 {
-    return bufs + 2 * (c >> 1) * cq_line_stride(nzz) + (c & 1);
+    return bufs + 2 * (c >> 1) * LS + (c & 1);
 }
 
 template <int CW>
@@ -229,7 +228,7 @@ __device__ __forceinline__ void col_fill(float *bufs, const JobG &J, const float
         },
         [&](int e, float4 v) {
             const int zq = e / CW, c = e - zq * CW;
-            float *line = col_line(bufs, nzz, c);
+            float *line = col_line(bufs, J.lsz, c);
             if (x0 + c >= J.nxx) v = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = 0; k < 4; k++)
@@ -276,7 +275,7 @@ __device__ __forceinline__ void col_vz_phase(const JobG &J, float *bufs, int x0)
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
             if (ix >= J.nxx) return;
             const size_t g = cq_q4(J.nxx, ix, zq);
-            float *line = col_line(bufs, nzz, c);
+            float *line = col_line(bufs, J.lsz, c);
             float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -324,7 +323,7 @@ __device__ __forceinline__ void col_div_phase(const JobG &J, float *bufs, int x0
         },
         [&](int e, const F4x2 &r) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-            float *line = col_line(bufs, nzz, c);
+            float *line = col_line(bufs, J.lsz, c);
             if (ix >= J.nxx) {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
@@ -376,7 +375,7 @@ __device__ __forceinline__ void col_extract(const JobG &J, float2 *buf, int x0, 
         [&](int e, float4 m) {
             const int k = e / L, l = e - k * L;
             const int k2 = k ? nzz - k : 0;
-            float2 *line = buf + l * cq_line_stride(nzz);
+            float2 *line = buf + l * J.lsz;
             const float2 zk = line[k], zm = line[k2];
             const float2 g0 = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             const float2 g1 = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
@@ -417,7 +416,7 @@ __device__ __forceinline__ void col_insert(const JobG &J, float2 *buf, int x0, c
             if (x0 + 2 * l >= J.nxx) h = make_float4(0.f, 0.f, 0.f, 0.f);
             if (x0 + 2 * l + 1 >= J.nxx) h.z = h.w = 0.f;
             float2 h0 = make_float2(h.x, h.y), h1 = make_float2(h.z, h.w);
-            float2 *line = buf + l * cq_line_stride(nzz);
+            float2 *line = buf + l * J.lsz;
             if (k2 == k) {
                 h0.y = 0.f;
                 h1.y = 0.f;
@@ -513,7 +512,7 @@ template <int TK>
 __device__ __forceinline__ void krow_fill(float2 *buf, const JobG &J, const float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
     constexpr int P = 4 * TK;
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     cq_loop<4>(
         ((nxx + 7) >> 3) * P,
         [&](int e) {
@@ -534,7 +533,7 @@ template <int TK>
 __device__ __forceinline__ void krow_store(const float2 *buf, const JobG &J, float2 CQ_GLOBAL *__restrict__ S, int k0)
 {
     constexpr int P = 4 * TK;
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     const int total = ((nxx + 7) >> 3) * P;
     for (int e = threadIdx.x; e < total; e += CQ_NT) {
         const int b = e / P, w = e - b * P;
@@ -594,7 +593,7 @@ __device__ __forceinline__ float aux_symbol_of(const JobG &J, int which, float k
 template <int TK>
 __device__ __forceinline__ void krow_round1(const JobG &J, float2 *buf, int k0, int which, float tt)
 {
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
@@ -611,7 +610,7 @@ __device__ __forceinline__ void krow_round1(const JobG &J, float2 *buf, int k0, 
 template <int TK>
 __device__ __forceinline__ void krow_round2(const JobG &J, float2 *buf, int k0)
 {
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
@@ -628,7 +627,7 @@ __device__ __forceinline__ void krow_round2(const JobG &J, float2 *buf, int k0)
 template <int TK>
 __device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst, int k0, int which)
 {
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
@@ -642,7 +641,7 @@ __device__ __forceinline__ void krow_mult_to(const JobG &J, float2 *buf, int dst
 template <int TK>
 __device__ __forceinline__ void krow_mult(const JobG &J, float2 *buf, int k0, int which, float tt = 0.f)
 {
-    const int nxx = J.nxx, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, LS = J.lsx;
     const float inv = __builtin_amdgcn_rcpf((float)nxx);
     for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
         const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx;
@@ -716,11 +715,11 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_row_ac(const JobD 
 {
     constexpr int L = TZ / 2, Q = TZ / 4;
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, z0 = it.y, LS = cq_line_stride(nxx);
+    const int nxx = J.nxx, z0 = it.y, LS = J.lsx;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nxx;
-    load_tw(tw, J.twx, nxx);
-    row_fill<Q, true>(buf, nxx, J.div, z0);
+    float2 *tw = lds, *buf = lds + J.twlx;
+    load_tw(tw, J.twx, J.twlx);
+    row_fill<Q, true>(buf, nxx, J.lsx, J.div, z0);
     __syncthreads();
     cq_fft_fwd(Jr.px, buf, LS, L, tw, J.mx2);
     cq_fft_inv(Jr.px, buf, LS, L, tw);
@@ -804,27 +803,27 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_col_ac(const JobD 
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nzz;
+    float2 *tw = lds, *buf = lds + J.twlz;
     float *bufs = reinterpret_cast<float *>(buf);
-    load_tw(tw, J.twz, nzz);
+    load_tw(tw, J.twz, J.twlz);
     col_fill<CW>(bufs, J, J.p, x0);
     __syncthreads();
     // receiver gather of the updated p (acps4.cu:611-621)
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzp);
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, J.lsz, c)[2 * z]; });
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, J.mzp);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, J.mzm);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     col_div_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mz2);
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, J.mz2);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
+        const float *line = col_line(bufs, J.lsz, c);
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -852,14 +851,14 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c1(const JobD *_
 {
     constexpr int L = CW / 2;
     CQ_ITEM_PROLOGUE
-    const int nzz = J.nzz, x0 = it.y;
+    const int x0 = it.y;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nzz;
+    float2 *tw = lds, *buf = lds + J.twlz;
     float *bufs = reinterpret_cast<float *>(buf);
-    load_tw(tw, J.twz, nzz);
+    load_tw(tw, J.twz, J.twlz);
     col_fill<CW>(bufs, J, J.div, x0);
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -868,10 +867,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r1(const JobD *_
                                                 int step)
 {
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, k0 = it.y, LS = cq_line_stride(nxx);
+    const int k0 = it.y, LS = J.lsx;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nxx;
-    load_tw(tw, J.twx, nxx);
+    float2 *tw = lds, *buf = lds + J.twlx;
+    load_tw(tw, J.twx, J.twlx);
     // A = |k|^(1/16) div, K = |k|^2 div, B = |k|^(33/16) div (acsglw.cu:553-566);
     // aux = |k|^1.01 p of the previous step (acsglw.cu:623-631)
     const int aux_symbol = J.stab ? 4 : 3;
@@ -928,9 +927,9 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nzz;
+    float2 *tw = lds, *buf = lds + J.twlz;
     float *bufs = reinterpret_cast<float *>(buf);
-    load_tw(tw, J.twz, nzz);
+    load_tw(tw, J.twz, J.twlz);
     // total = y1 div + y2 A - (dt^2 v^2 lam/12)(y3 K + y4 B), with
     // y1 = 1-32 gam, y2 = 32 gam vwd, y3 = 1-64 gam, y4 = 64 gam vwd = 2 y2.
     if constexpr (CW >= 24) {
@@ -941,7 +940,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         // of a cell is formed in LDS; P of column pairs is then packed into
         // the even lines, A goes into the odd lines, total = y2 A - P.
         constexpr int NC = CW / 2, NP = NC / 2;
-        const int LS = cq_line_stride(nzz);
+        const int LS = J.lsz;
         // item -> (quad, column) by the reciprocal of an opaque NC: with a
         // compile-time power of two the compiler hoists per-thread offsets
         // of every loop out of the half loop and runs out of registers
@@ -1028,7 +1027,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     // `tot` itself; each thread re-reads only the elements it wrote.
     col_insert<CW>(J, buf, x0, J.s3);   // K
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     // column c of the tile: ix = x0 + c; quads beyond nxx are skipped
     auto cell = [&](int e, int &zq, int &c) {
         zq = e / CW;
@@ -1046,7 +1045,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         [&](int e, float4 gm) {
             int zq, c;
             if (!cell(e, zq, c)) return;
-            const float *line = col_line(bufs, nzz, c);
+            const float *line = col_line(bufs, J.lsz, c);
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -1058,7 +1057,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s4);   // B
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     cq_loop<4, true>(
         CW * nq,
         [&](int e) {
@@ -1075,7 +1074,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         [&](int e, const F4x3 &r) {
             int zq, c;
             if (!cell(e, zq, c)) return;
-            const float *line = col_line(bufs, nzz, c);
+            const float *line = col_line(bufs, J.lsz, c);
             const float4 y2 = r.a, cc = r.b;
             float4 a = r.c;
 #pragma unroll
@@ -1088,7 +1087,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s1);   // A
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     cq_loop<3, true>(
         CW * nq,
         [&](int e) {
@@ -1106,7 +1105,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
         [&](int e, const F4x5 &r) {
             int zq, c;
             const bool in = cell(e, zq, c);
-            float *line = col_line(bufs, nzz, c);
+            float *line = col_line(bufs, J.lsz, c);
             if (!in) {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
@@ -1127,7 +1126,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *_
             st4(J.tot + cq_q4(J.nxx, x0 + c, zq), o);
         });
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s1, nullptr);
 }
 
@@ -1136,16 +1135,16 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r2(const JobD *_
                                                 int step)
 {
     CQ_ITEM_PROLOGUE
-    const int nxx = J.nxx, k0 = it.y;
+    const int k0 = it.y;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nxx;
-    load_tw(tw, J.twx, nxx);
+    float2 *tw = lds, *buf = lds + J.twlx;
+    load_tw(tw, J.twx, J.twlx);
     krow_fill<TK>(buf, J, J.s1, k0);
     __syncthreads();
-    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), TK, tw, nullptr);
+    cq_fft_fwd(Jr.px, buf, J.lsx, TK, tw, nullptr);
     krow_mult<TK>(J, buf, k0, 3);   // T = |k|^1.01 total (acsglw.cu:578-588)
     __syncthreads();
-    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), TK, tw);
+    cq_fft_inv(Jr.px, buf, J.lsx, TK, tw);
     krow_store<TK>(buf, J, J.s1, k0);
 }
 
@@ -1157,17 +1156,17 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     CQ_ITEM_PROLOGUE
     const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nzz;
+    float2 *tw = lds, *buf = lds + J.twlz;
     float *bufs = reinterpret_cast<float *>(buf);
-    load_tw(tw, J.twz, nzz);
+    load_tw(tw, J.twz, J.twlz);
     col_insert<CW>(J, buf, x0, J.s1);   // T
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     // T waits in the scratch field (each thread re-reads only what it wrote)
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
+        const float *line = col_line(bufs, J.lsz, c);
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1179,7 +1178,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     __syncthreads();
     col_insert<CW>(J, buf, x0, J.s2);   // aux of the previous step
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     // p += rho v^2 dt lam (total - s dt tao T / 2) - s dt tao aux; source.
     // s = +1 attenuates (acsglw.cu:1500-1537); s = -1 compensates (Q-RTM,
     // sgpstd-qrtm/acsglw.cu:1769-1841).  The new p replaces aux in the
@@ -1207,7 +1206,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
         },
         [&](int e, const C3In &r) {
             const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
-            float *line = col_line(bufs, nzz, c);
+            float *line = col_line(bufs, J.lsz, c);
             if (ix >= J.nxx) {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
@@ -1230,13 +1229,13 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
         // sgpstd-qrtm/acsglw.cu:1809-1841)
         const int tile = x0 / CW;
         for (int r = J.inj_off[tile] + threadIdx.x; r < J.inj_off[tile + 1]; r += CQ_NT)
-            col_line(bufs, nzz, J.inj_x[r] - x0)[2 * J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
+            col_line(bufs, J.lsz, J.inj_x[r] - x0)[2 * J.inj_z[r]] = J.obs[(size_t)t * J.nr + J.inj_i[r]];
         __syncthreads();
     }
     for (int e = threadIdx.x; e < CW * nq; e += CQ_NT) {
         const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
         if (ix >= J.nxx) continue;
-        const float *line = col_line(bufs, nzz, c);
+        const float *line = col_line(bufs, J.lsz, c);
         const size_t g = cq_q4(J.nxx, ix, zq);
         float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -1246,16 +1245,16 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
         if (rcv) interior_accum(J, J.img, ix, 4 * zq, pn, ld4(J.psrc + g));
         else if (flags & CQ_F_SSG) interior_accum(J, J.ssg, ix, 4 * zq, pn, pn);
     }
-    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, nzz, c)[2 * z]; });
+    if (J.gather) col_gather<CW>(J, x0, t, [&](int c, int z) { return col_line(bufs, J.lsz, c)[2 * z]; });
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, nullptr);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, nullptr);
     col_extract<CW>(J, buf, x0, J.s2, J.mzp);   // F_z p for the next aux; x dp/dz multiplier
     __syncthreads();
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     col_vz_phase<CW>(J, bufs, x0);
     __syncthreads();
-    cq_fft_fwd(Jr.pz, buf, cq_line_stride(nzz), L, tw, J.mzm);
-    cq_fft_inv(Jr.pz, buf, cq_line_stride(nzz), L, tw);
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, J.mzm);
+    cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
     col_div_phase<CW, true>(J, bufs, x0);
 }
 
@@ -1267,15 +1266,15 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r3(const JobD *_
     CQ_ITEM_PROLOGUE
     const int nxx = J.nxx, z0 = it.y;
     float2 *const lds = cq_lds;
-    float2 *tw = lds, *buf = lds + nxx;
-    load_tw(tw, J.twx, nxx);
-    row_fill<Q>(buf, nxx, J.p, z0);
+    float2 *tw = lds, *buf = lds + J.twlx;
+    load_tw(tw, J.twx, J.twlx);
+    row_fill<Q>(buf, nxx, J.lsx, J.p, z0);
     __syncthreads();
-    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), L, tw, J.mxp);
-    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), L, tw);
+    cq_fft_fwd(Jr.px, buf, J.lsx, L, tw, J.mxp);
+    cq_fft_inv(Jr.px, buf, J.lsx, L, tw);
     row_vx_phase<Q>(J, buf, z0);
     __syncthreads();
-    cq_fft_fwd(Jr.px, buf, cq_line_stride(nxx), L, tw, J.mxm);
-    cq_fft_inv(Jr.px, buf, cq_line_stride(nxx), L, tw);
+    cq_fft_fwd(Jr.px, buf, J.lsx, L, tw, J.mxm);
+    cq_fft_inv(Jr.px, buf, J.lsx, L, tw);
     row_dvx_phase<Q, true>(J, buf, z0);
 }

@@ -28,7 +28,7 @@ typedef enum {
     CQ_EINVAL = -1,        /* bad argument */
     CQ_EDEVICE = -2,       /* HIP runtime error */
     CQ_ENOMEM = -3,        /* device allocation failed */
-    CQ_EUNSUPPORTED = -4,  /* e.g. an FFT length with an unsupported prime factor */
+    CQ_EUNSUPPORTED = -4,  /* e.g. a window whose transforms exceed one workgroup's LDS */
     CQ_ESTATE = -5,        /* call out of order (e.g. reading results before running) */
     CQ_ENUMERIC = -6       /* non-finite wavefield detected */
 } cq_status;

@@ -18,9 +18,11 @@
  *   viscoacoustic   : sgpstd-qmodeling/acsglw.cu:511-683 (kernels :1213-1245,
  *                     :1500-1537)
  *   RTM / Q-RTM     : sgpstd-qrtm/acps4.cu:447-945, sgpstd-qrtm/acsglw.cu:470-1079
- * The FFT is this file's own mixed-radix transform (no FFTW in the image): the
- * unnormalised DFT with exp(-i) forward and exp(+i) inverse, which is what the
- * reference's cufftExecC2C computes (third-party cuFFT, version unpinned).
+ * The FFT is this file's own mixed-radix transform (no FFTW in the image; prime
+ * factors above 64 by Bluestein's chirp-z algorithm over a power-of-two
+ * length): the unnormalised DFT with exp(-i) forward and exp(+i) inverse,
+ * which is what the reference's cufftExecC2C computes (third-party cuFFT,
+ * version unpinned).
  *
  * Coefficient arrays (CPML profiles, shift multipliers, |k| powers, windowed
  * models) are inputs: the tests pass the FP32 arrays the reference's own host
@@ -48,18 +50,31 @@ typedef struct { real re, im; } cpx;
 /* Mixed-radix FFT                                                           */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
+typedef struct fplan_s {
     int n;
     int nf;
     int fac[ORC_MAXFAC];
     cpx *tw; /* tw[j] = exp(-2*pi*i*j/n), computed in double */
+    /* lengths with a prime factor above ORC_BLUE_PRIME: Bluestein's chirp-z
+     * transform over M = the smallest power of two >= 2n - 1 points */
+    int M;
+    struct fplan_s *pm;   /* plan of the M-point transforms */
+    cpx *chirp;           /* exp(-i pi j^2 / n), j < n */
+    cpx *bspec;           /* FFT_M of the conjugate chirp wrapped cyclically, / M */
 } fplan;
+
+#define ORC_BLUE_PRIME 64
+
+static void fft_line(const struct fplan_s *pl, const cpx *in, int xs, cpx *out, int inv);
 
 static void plan_init(fplan *p, int n)
 {
     int m = n, f;
     p->n = n;
     p->nf = 0;
+    p->M = 0;
+    p->pm = NULL;
+    p->chirp = p->bspec = NULL;
     /* radix 4 first, then 2, 3, 5, then odd factors in increasing order */
     while (m % 4 == 0) { p->fac[p->nf++] = 4; m /= 4; }
     while (m % 2 == 0) { p->fac[p->nf++] = 2; m /= 2; }
@@ -72,9 +87,47 @@ static void plan_init(fplan *p, int n)
         p->tw[j].re = (real)cos(a);
         p->tw[j].im = (real)sin(a);
     }
+    if (p->fac[p->nf - 1] > ORC_BLUE_PRIME) {
+        /* X[k] = c[k] sum_j (x[j] c[j]) conj c[k - j], c[j] = exp(-i pi j^2 / n):
+         * a linear convolution of n-point sequences, done cyclically on M points */
+        int M = 1;
+        while (M < 2 * n - 1) M *= 2;
+        p->M = M;
+        p->pm = (fplan *)malloc(sizeof(fplan));
+        plan_init(p->pm, M);
+        p->chirp = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+        p->bspec = (cpx *)malloc(sizeof(cpx) * (size_t)M);
+        cpx *b = (cpx *)calloc((size_t)M, sizeof(cpx));
+        for (int j = 0; j < n; j++) {
+            const long long r = (long long)j * j % (2LL * n);   /* exact phase reduction */
+            const double a = -M_PI * (double)r / (double)n;
+            p->chirp[j].re = (real)cos(a);
+            p->chirp[j].im = (real)sin(a);
+            b[j].re = (real)cos(a);
+            b[j].im = (real)-sin(a);
+            if (j) b[M - j] = b[j];
+        }
+        fft_line(p->pm, b, 1, p->bspec, 0);
+        for (int k = 0; k < M; k++) {
+            p->bspec[k].re /= (real)M;
+            p->bspec[k].im /= (real)M;
+        }
+        free(b);
+    }
 }
 
-static void plan_free(fplan *p) { free(p->tw); p->tw = NULL; }
+static void plan_free(fplan *p)
+{
+    free(p->tw);
+    p->tw = NULL;
+    if (p->pm) {
+        plan_free(p->pm);
+        free(p->pm);
+        free(p->chirp);
+        free(p->bspec);
+        p->pm = NULL;
+    }
+}
 
 /* Recursive decimation in time: x has stride xs, X is contiguous.  At a level
  * of length n the twiddle stride into the top-level table is tws = N/n; the
@@ -108,9 +161,39 @@ static void fft_rec_top(const cpx *x, int xs, cpx *X, int n, const int *fac,
     }
 }
 
-/* One line: in (stride xs) -> out (contiguous). */
+/* One line: in (stride xs) -> out (contiguous).  An inverse Bluestein
+ * transform is the conjugate of the forward transform of the conjugate. */
 static void fft_line(const fplan *pl, const cpx *in, int xs, cpx *out, int inv)
 {
+    if (pl->pm) {
+        const int n = pl->n, M = pl->M;
+        cpx *a = (cpx *)calloc((size_t)M * 2, sizeof(cpx)), *A = a + M;
+        for (int j = 0; j < M; j++) {
+            if (j < n) {
+                cpx x = in[(size_t)j * xs], c = pl->chirp[j];
+                if (inv) x.im = -x.im;
+                a[j].re = x.re * c.re - x.im * c.im;
+                a[j].im = x.re * c.im + x.im * c.re;
+            } else {
+                a[j].re = a[j].im = 0;
+            }
+        }
+        fft_line(pl->pm, a, 1, A, 0);
+        for (int k = 0; k < M; k++) {
+            const cpx u = A[k], w = pl->bspec[k];
+            A[k].re = u.re * w.re - u.im * w.im;
+            A[k].im = u.re * w.im + u.im * w.re;
+        }
+        fft_line(pl->pm, A, 1, a, 1);
+        for (int k = 0; k < n; k++) {
+            const cpx r = a[k], c = pl->chirp[k];
+            out[k].re = r.re * c.re - r.im * c.im;
+            out[k].im = r.re * c.im + r.im * c.re;
+            if (inv) out[k].im = -out[k].im;
+        }
+        free(a);
+        return;
+    }
     cpx tmp[ORC_MAXRADIX > 64 ? 64 : ORC_MAXRADIX];
     cpx *t = tmp;
     int big = 0;

@@ -112,6 +112,10 @@ def test_bad_inputs_raise(ctx):
     with pytest.raises(engine.CqError):
         ctx.batch([bad])
     bad = dict(job)
-    bad["nxx"] = 61 * 2  # not matching arrays but also a prime factor 61
+    bad["pml"] = 30          # CPML strips wider than the window
+    with pytest.raises(engine.CqError):
+        ctx.batch([bad])
+    bad = dict(job)
+    bad["nt"] = 0
     with pytest.raises(engine.CqError):
         ctx.batch([bad])
