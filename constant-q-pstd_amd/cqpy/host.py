@@ -149,6 +149,21 @@ def projectomodel(win_interior, nxt, nzt, nx, nz, pml, mlx, dx, full=None):
     return full
 
 
+def filter_image(img, nx, nz, dx, dz):
+    """5-point Laplacian of a raw interior image, zero border (sgpstd-qrtm/acps4.cu:1723-1745)."""
+    img = np.ascontiguousarray(img, np.float32).ravel()
+    out = np.zeros_like(img)
+    lib().cqh_filter_image(C.c_int(nx), C.c_int(nz), C.c_float(dx), C.c_float(dz), _fp(img), _fp(out))
+    return out
+
+
+def mute_image(img, nx, nz, pml, lgest, rgest):
+    """Zero interior columns outside the receiver span (sgpstd-qrtm/acps4.cu:926-935)."""
+    img = np.ascontiguousarray(img, np.float32).ravel().copy()
+    lib().cqh_mute_image(C.c_int(nx), C.c_int(nz), C.c_int(pml), C.c_int(lgest), C.c_int(rgest), _fp(img))
+    return img
+
+
 def removedirect(obs, tmax, nr, dt, fd, xs, zs, gxp, gzp, v0):
     obs = np.ascontiguousarray(obs, np.float32).ravel().copy()
     gxp = np.ascontiguousarray(gxp, np.float32)

@@ -1519,25 +1519,24 @@ static const int kCheckEvery = 256;
 // per-stage capacity of the power-of-two radices.
 static int server_tiles(cq_batch b, int mx, int mz)
 {
-    const int cap = cq_stage_capacity(16);
     int iz = 0, ic = 0, ik = 0;
-    auto lim = [&](size_t lds, int pts, size_t budget) { return lds > budget || pts > cap; };
-    auto r1_bad = [&](int k, size_t budget) {
-        return lim(lds_lines(mx, 3 * k), 2 * k * mx, budget) && lim(lds_lines(mx, 2 * k), k * mx, budget);
-    };
-    while (iz < 3 && lim(lds_lines(mx, kTzCand[iz] / 2), (kTzCand[iz] / 2) * mx, kLdsTile)) iz++;
-    while (ic < 6 && lim(lds_lines(mz, kCwCand[ic] / 2), (kCwCand[ic] / 2) * mz, kLdsTile)) ic++;
+    // tiles of `lines` lines of the largest window's length n: bad if over
+    // the LDS budget or if no plan holds that many points (a job with a
+    // shorter length whose plan does not fit is refused when submitted)
+    auto lim = [&](int n, int lines, size_t budget) { return lds_lines(n, lines) > budget || !has_plan(n, lines * n); };
+    auto r1_bad = [&](int k, size_t budget) { return lim(mx, 3 * k, budget) && lim(mx, 2 * k, budget); };
+    while (iz < 3 && lim(mx, kTzCand[iz] / 2, kLdsTile)) iz++;
+    while (ic < 6 && lim(mz, kCwCand[ic] / 2, kLdsTile)) ic++;
     while (ik < 3 && r1_bad(kTkCand[ik], kLdsTile)) ik++;
     b->tz = kTzCand[iz];
     b->cw = kCwCand[ic];
     b->cwq = kCwCand[ic];
     b->tk = kTkCand[ik];
-    if (lim(lds_lines(mx, b->tz / 2), (b->tz / 2) * mx, kLdsMax) || lim(lds_lines(mz, b->cw / 2), (b->cw / 2) * mz, kLdsMax) ||
-        r1_bad(b->tk, kLdsMax))
+    if (lim(mx, b->tz / 2, kLdsMax) || lim(mz, b->cw / 2, kLdsMax) || r1_bad(b->tk, kLdsMax))
         return set_err(CQ_EUNSUPPORTED, "server windows up to %dx%d are too large for one workgroup", mx, mz);
     b->tk2 = b->tk;
     for (int m : {3, 2})
-        if (m * b->tk <= 16 && !lim(lds_lines(mx, m * b->tk), m * b->tk * mx, kLdsTile)) {
+        if (m * b->tk <= 16 && !lim(mx, m * b->tk, kLdsTile)) {
             b->tk2 = m * b->tk;
             break;
         }

@@ -1,0 +1,137 @@
+"""The Q-RTM and serving workloads at their real sizes, through the C ABI.
+
+Q-RTM (800 x 3000 model, 40-point CPML, natural windows 2081 x 880 with
+2081 a prime, 40 checkpoints): where the FP64 oracle is too slow for the full
+run, size-independent properties pin the GPU result -- the image does not
+depend on the revolve schedule (40 or 64 checkpoints, bit for bit, with the
+wavelet's first sample zero so that the reference's checkpoint-0 quirk,
+sgpstd-qrtm/acps4.cu:468-475, does not apply), it is linear in the data
+(doubled data give exactly the doubled image) and finite -- and a short run
+of the same window is compared with the oracle.  A long compensated Q-RTM
+(nt = 3000, the regime where FP32 drift would show) runs against the FP64
+oracle on a small window.
+
+Serving: the largest window of the serving workload (4200 x 1120 padded,
+1024 x 4096 cells plus CPML rounded to 7-smooth lengths) in a server next to
+small jobs: each result equals the job served alone, bit for bit."""
+import numpy as np
+import pytest
+
+import orc
+from cqpy import engine, host, synth
+from test_gpu_rtm import rtm_job
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+def qrtm_shot(kind, nt, snaps, widen=False):
+    j = synth.qrtm_jobs(nshots=1, nt=nt, snaps=snaps, kind=kind, widen=widen)[0]
+    j["rik"] = j["rik"].copy()
+    j["rik"][0] = 0.0
+    return j
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_config4_image_independent_of_schedule_and_linear(ctx, kind):
+    nt = 300
+    a = qrtm_shot(kind, nt, 40)
+    b = dict(a, snaps=64)
+    c = dict(a, obs=2.0 * a["obs"])
+    print("case kind", kind, "window", a["nxx"], a["nzz"], "nt", nt, flush=True)
+    bt = ctx.batch([a, b, c])
+    bt.run()
+    ia, sa = bt.image(0)
+    ib, sb = bt.image(1)
+    ic, sc = bt.image(2)
+    bt.close()
+    assert np.isfinite(ia).all() and np.isfinite(sa).all()
+    assert np.abs(ia).max() > 0 and sa.max() > 0
+    assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
+    assert np.array_equal(ic, 2.0 * ia) and np.array_equal(sc, sa)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_config4_window_vs_oracle(ctx, kind):
+    nt, snaps = 8, 3
+    job = qrtm_shot(kind, nt, snaps)
+    job.update(host.k2d(job["dx"], job["dz"], job["nxx"], job["nzz"], q=True))
+    job["rik"][0] = 0.3          # the quirk applies here
+    print("case kind", kind, "window", job["nxx"], job["nzz"], flush=True)
+    ref_img, ref_ssg = orc.rtm(job, host.revolve_trace(nt, snaps)[0])
+    b = ctx.batch([job])
+    b.run()
+    img, ssg = b.image(0)
+    b.close()
+    ei, es = engine.rel_l2(img, ref_img), engine.rel_l2(ssg, ref_ssg)
+    print("rel-L2 image", ei, "illumination", es)
+    assert es <= 1e-4 and ei <= 1e-3
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_long_compensated_rtm_vs_fp64(ctx, kind):
+    """nt = 3000 with 20 checkpoints (12976 executed steps), stabiliser
+    1e-20 on both wavefields (effective sigma^2 = 1e-26): the FP32 image and
+    illumination (both halves of QRTM.BIN) stay within the image tolerance
+    1e-3 of the FP64 oracle's and finite.  Measured: image 1.2e-6 for both
+    kinds; illumination 6e-6 (RTM) and 1.4e-4 (Q-RTM: the compensated source
+    wavefield's FP32 drift over 3000 steps, which the sum of p^2 weights
+    toward the late, amplified times)."""
+    nt, snaps = 3000, 20
+    job = rtm_job(kind, nt=nt, snaps=snaps, seed=61)
+    print("case kind", kind, "window", job["nxx"], job["nzz"], "nt", nt, flush=True)
+    ref_img, ref_ssg = orc.rtm(job, host.revolve_trace(nt, snaps)[0])
+    b = ctx.batch([job])
+    b.run()
+    img, ssg = b.image(0)
+    b.close()
+    assert np.isfinite(img).all()
+    ei, es = engine.rel_l2(img, ref_img), engine.rel_l2(ssg, ref_ssg)
+    print("rel-L2 image", ei, "illumination", es)
+    assert es <= 1e-3 and ei <= 1e-3
+
+
+def big_job(kind, nt, seed):
+    nx, nz, pml = 4200 - 80, 1120 - 80, 40
+    vp, rho, q = synth.layered_model(nx, nz, 5.0, seed=seed, vmin=1500.0, vmax=4500.0, qrange=(30.0, 300.0),
+                                     noise=0.0)
+    dt = float(np.float32(0.4 * 5.0 / 4500.0))
+    return synth.make_window_job(kind, vp, rho, nx, nz, 5.0, 5.0, dt, nt, 20.0, 0, nx, nx // 2, 2,
+                                 np.arange(0, nx, 4), 2, pml, qt=q, fo=20.0)
+
+
+def test_serve_largest_window_alone_equals_mix(ctx):
+    MX, MZ = 4200, 1120
+    jobs = [big_job(0, 60, 71), big_job(1, 50, 72),
+            synth.small_job(nxt=40, nzt=24, pml=5, nt=120, kind=0, seed=73),
+            rtm_job(3, nt=80, snaps=5, seed=74)]
+    assert (jobs[0]["nxx"], jobs[0]["nzz"]) == (MX, MZ)
+
+    def serve(js, stagger):
+        s = ctx.server(MX, MZ)
+        ids = []
+        for j in js:
+            ids.append(s.submit(j))
+            if stagger:
+                s.advance(7)
+        while any(s.state(i) in (engine.QUEUED, engine.RUNNING) for i in ids):
+            s.advance(25)
+        out = [s.result(i) for i in ids]
+        s.close()
+        return out
+
+    alone = [serve([j], False)[0] for j in jobs]
+    mixed = serve(jobs, True)
+    for k in range(len(jobs)):
+        a, m = alone[k], mixed[k]
+        if isinstance(a, tuple):
+            assert all(np.array_equal(x, y) for x, y in zip(a, m)), k
+        else:
+            assert np.isfinite(a).all() and np.abs(a).max() > 0
+            assert np.array_equal(a, m), k

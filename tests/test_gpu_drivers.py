@@ -3,7 +3,9 @@ parameter files, geometry files and float32 models in, reference-named
 outputs out.  aps / vaps gathers match the FP64 oracle on the same shots
 (rel-L2 <= 1e-4), snapshots (flagout = 1) have the reference's layout and
 equal the engine's pressure field, and artm / qrtm write [image |
-illumination] stacks of the model size."""
+illumination] stacks equal to the ones assembled from the FP64 oracle.  The
+reference's own shipped parameter files and geometry run unchanged but for
+the time axis (test_shipped_default_run)."""
 import os
 import subprocess
 
@@ -11,7 +13,7 @@ import numpy as np
 import pytest
 
 import orc
-from cqpy import engine, synth
+from cqpy import engine, host, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -107,16 +109,112 @@ def test_modeling_driver_vs_oracle(tree, mode, kind, prefix):
     assert e <= 1e-5
 
 
-@pytest.mark.parametrize("mode,out", [("artm", "RTM.BIN"), ("qrtm", "QRTM.BIN")])
-def test_rtm_driver_writes_stack(tree, mode, out):
+def expected_stack(t, kind, vp, rho, q, snaps, v0, eta):
+    """The [image | illumination] stack the reference's RTM driver builds
+    (sgpstd-qrtm/acps4.cu:909-945, main.c:470-480), assembled here from the
+    FP64 oracle's raw image of each shot: observed gather read back from the
+    modeling run, direct wave removed, Laplacian filter, mute outside the
+    receiver span, projection to the model, sum over shots."""
+    img_sp = np.zeros(NXT * NZT, np.float32)
+    ssg_sp = np.zeros(NXT * NZT, np.float32)
+    nr = 2 * HALF + 1
+    for i, s in enumerate(SHOTS):
+        job = shot_job(kind, vp, rho, q, s)
+        gxp = (np.arange(s - HALF, s + HALF + 1) * DX).astype(np.float32)
+        gzp = np.full(nr, 2 * DX, np.float32)
+        obs = np.fromfile(t / "obsdata" / ("%s%d_%d.bin" % ("reca" if kind == 2 else "recva", i, nr)), np.float32)
+        obs = host.removedirect(obs, NT, nr, DT_MS / 1000.0, FD, s * DX, 2 * DX, gxp, gzp, v0).reshape(NT, nr)
+        job.update(obs=obs, snaps=snaps, etas=eta, etar=eta)
+        img, ssg = orc.rtm(job, host.revolve_trace(NT, snaps)[0])
+        nx, nz = job["nxx"] - 2 * PML, job["nzz"] - 2 * PML
+        imgf = host.filter_image(img.astype(np.float32).ravel(), nx, nz, DX, DX)
+        imgf = host.mute_image(imgf, nx, nz, PML, job["lgest"], job["rgest"])
+        # projectomodel writes the window into a zeroed model, then the stack adds it
+        img_sp += host.projectomodel(imgf, NXT, NZT, nx, nz, PML, job["mlx"], DX)
+        ssg_sp += host.projectomodel(ssg.astype(np.float32).ravel(), NXT, NZT, nx, nz, PML, job["mlx"], DX)
+    return img_sp, ssg_sp
+
+
+@pytest.mark.parametrize("mode,out,kind", [("artm", "RTM.BIN", 2), ("qrtm", "QRTM.BIN", 3)])
+def test_rtm_driver_stack_vs_oracle(tree, mode, out, kind):
+    """artm / qrtm write [image | illumination] of the model size; both equal
+    the stack assembled from the FP64 oracle (image rel-L2 <= 1e-3,
+    illumination <= 1e-4).  The driver runs the three shots two at a time."""
     t, vp, rho, q = tree
     run(t, "aps" if mode == "artm" else "vaps")   # observed data
     run(t, mode)
-    img = np.fromfile(t / "work" / out, np.float32)
-    assert img.size == 2 * NXT * NZT
-    img, ssg = img[:NXT * NZT], img[NXT * NZT:]
+    got = np.fromfile(t / "work" / out, np.float32)
+    assert got.size == 2 * NXT * NZT
+    img, ssg = got[:NXT * NZT], got[NXT * NZT:]
     assert np.isfinite(img).all() and np.isfinite(ssg).all()
-    assert np.abs(img).max() > 0 and ssg.min() >= 0 and ssg.max() > 0
+    ref_img, ref_ssg = expected_stack(t, kind, vp, rho, q, 12, 1500.0, 1e-20)
+    ei, es = engine.rel_l2(img, ref_img), engine.rel_l2(ssg, ref_ssg)
+    print(mode, "stack rel-L2 vs oracle: image", ei, "illumination", es)
+    assert es <= 1e-4
+    assert ei <= 1e-3
+
+
+# ---------------------------------------------------------------------------
+# the reference's shipped inputs, unchanged but for the time axis
+# ---------------------------------------------------------------------------
+SHIP_NT = 500
+
+
+@pytest.fixture(scope="module")
+def shipped(tmp_path_factory):
+    """The reference's directory layout with its shipped parameter files
+    (tests/golden/input, only the time sample number cut from 2501 to
+    SHIP_NT) and its geometry generator's parameters
+    (tests/golden/geometry_parameters.txt: 80 shots from 400 m every 40 m,
+    1500 m offset), run through bin/ge; the 398 x 160 models are synthetic
+    (seeded layered vp up to 3500 m/s, Gardner density, Q 30-200)."""
+    t = tmp_path_factory.mktemp("ship")
+    for d in ("work", "input", "geometry", "output", "obsdata"):
+        (t / d).mkdir()
+    (t / "geometry" / "parameters.txt").write_text(
+        open(os.path.join(ROOT, "tests", "golden", "geometry_parameters.txt")).read())
+    r = subprocess.run([os.path.join(BIN, "ge"), "parameters.txt"], cwd=t / "geometry", capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vp, rho, q = synth.layered_model(398, 160, 10.0, seed=61, vmin=1500.0, vmax=3500.0, qrange=(30.0, 200.0))
+    vp.tofile(t / "output" / "gasv160_398.bin")
+    rho.tofile(t / "output" / "rho160_398.bin")
+    q.tofile(t / "output" / "gasq160_398.bin")
+    for name in ("ac2drealmodeling.txt", "visac2drealmodeling.txt"):
+        lines = open(os.path.join(GOLD, name)).read().split("\n")
+        k = [i for i, l in enumerate(lines) if l.startswith("--time sample number")][0]
+        lines[k + 1] = "  %d" % SHIP_NT
+        (t / "input" / name).write_text("\n".join(lines))
+    return t, vp, rho, q
+
+
+@pytest.mark.parametrize("mode,kind,prefix,shots", [("aps", 0, "reca", (0, 79)), ("vaps", 1, "recva", (40,))])
+def test_shipped_default_run(shipped, mode, kind, prefix, shots):
+    """bin/aps and bin/vaps with no arguments read ../input/<reference
+    name>.txt, size the reference's window (301 x 160, padded 321 x 180 with
+    its pml = 10; 321 = 3 107 runs by Bluestein), run all 80 shots and write
+    ../obsdata/<reca|recva><shot>_<nr>.bin; sampled shots match the FP64
+    oracle built from the same geometry files (rel-L2 <= 1e-4)."""
+    t, vp, rho, q = shipped
+    print("driver", mode, "default input", flush=True)
+    r = subprocess.run([os.path.join(BIN, mode)], cwd=t / "work", capture_output=True, text=True, timeout=280)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "window 301 x 160 (padded 321 x 180), 80 shots" in r.stdout, r.stdout[:400]
+    tracenum = np.fromfile(t / "geometry" / "tracenum.bin", np.int32)
+    shotxp = np.fromfile(t / "geometry" / "shotxp.bin", np.float32)
+    shotzp = np.fromfile(t / "geometry" / "shotzp.bin", np.float32)
+    assert tracenum.size == 80
+    for s in shots:
+        nr = int(tracenum[s])
+        got = np.fromfile(t / "obsdata" / ("%s%d_%d.bin" % (prefix, s, nr)), np.float32).reshape(SHIP_NT, nr)
+        gxp = np.fromfile(t / "geometry" / ("gxp%d.bin" % s), np.float32)
+        gzp = np.fromfile(t / "geometry" / ("gzp%d.bin" % s), np.float32)
+        job = synth.make_job(kind, vp, rho, 398, 160, 10.0, 10.0, 0.001, SHIP_NT, 30.0, float(shotxp[s]),
+                             float(shotzp[s]), gxp, gzp, 10, qt=q, fo=200.0, sroffmx=1500.0)
+        assert (job["nxx"], job["nzz"]) == (321, 180)
+        e = engine.rel_l2(got, orc.forward(job))
+        print(mode, "shot", s, "nr", nr, "rel-L2 vs FP64 oracle", e, flush=True)
+        assert e <= 1e-4
 
 
 @pytest.mark.parametrize("split,total", [("1,2", 2), ("2,1", 3)])
