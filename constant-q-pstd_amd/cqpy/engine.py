@@ -29,6 +29,7 @@ class JobDesc(C.Structure):
         ("rik", _F), ("xsg", C.c_int), ("zsg", C.c_int),
         ("nr", C.c_int), ("gxg", _I), ("gzg", _I),
         ("obs", _F), ("snaps", C.c_int), ("etas", C.c_float), ("etar", C.c_float),
+        ("lr_rank", C.c_int), ("lr_w", _F), ("lr_sym", _F),
     ]
 
 
@@ -38,6 +39,7 @@ class ServerDesc(C.Structure):
 
 
 QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+NPASS = 10   # CQ_NPASS
 
 SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_create",
            "cq_batch_step", "cq_batch_sync", "cq_batch_run", "cq_batch_read_rec",
@@ -105,6 +107,12 @@ def make_desc(job, keep):
         ob = np.ascontiguousarray(job["obs"], np.float32)
         keep.append(ob)
         d.obs = ob.ctypes.data_as(_F)
+    if int(job.get("lr_rank", 0)) > 0:
+        w = np.ascontiguousarray(job["lr_w"], np.float32)
+        sy = np.ascontiguousarray(job["lr_sym"], np.float32)
+        keep += [w, sy]
+        d.lr_rank = int(job["lr_rank"])
+        d.lr_w, d.lr_sym = w.ctypes.data_as(_F), sy.ctypes.data_as(_F)
     d.snaps = int(job.get("snaps", 0))
     d.etas = float(job.get("etas", 0.0))
     d.etar = float(job.get("etar", 0.0))
@@ -220,11 +228,11 @@ class Batch:
     def kernel_times(self):
         """{kernel name: (average ms per launch, launches)} for the passes
         timed while timing was enabled."""
-        ms = (C.c_double * 8)()
-        calls = (C.c_int * 8)()
+        ms = (C.c_double * NPASS)()
+        calls = (C.c_int * NPASS)()
         _chk(lib().cq_batch_kernel_times(self.h, ms, calls))
         out = {}
-        for p in range(8):
+        for p in range(NPASS):
             if calls[p]:
                 out[lib().cq_pass_name(C.c_int(p)).decode()] = (ms[p], calls[p])
         return out

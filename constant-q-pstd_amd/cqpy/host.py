@@ -37,6 +37,9 @@ def lib():
         L.cqh_check_stability.restype = C.c_float
         L.cqh_get_max_tao.restype = C.c_float
         L.cqh_window_sroffmx.restype = C.c_float
+        L.cqh_lr_symbol.restype = C.c_double
+        L.cqh_lr_symbol.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.cqh_lowrank_error.restype = C.c_double
         _lib = L
     return _lib
 
@@ -206,3 +209,68 @@ def revolve_trace(steps, snaps, limit=10_000_000):
         if a in (5, 6):
             break
     return out, r.info
+
+
+LR_TAYLOR4, LR_FRACLAP = 0, 1
+_D = C.POINTER(C.c_double)
+
+
+def lowrank_cc(vp, lam, dt):
+    """cc = dt^2 v^2 lam / 12 per cell, in float32 as the engine forms it."""
+    vp = np.asarray(vp, np.float32)
+    lam = np.asarray(lam, np.float32)
+    dtf = np.float32(dt) * np.float32(dt)
+    return (np.float32(1.0 / 12.0) * dtf * (vp * vp) * lam).astype(np.float32)
+
+
+def lowrank(job, rank, symbol=LR_FRACLAP, seed=1):
+    """FP64 low-rank decomposition of the viscoacoustic operator of a job
+    window (cqh_lowrank).  Returns (U [N1, r], A [r, r], V [r, Nq]) as
+    float64 arrays."""
+    nxx, nzz = int(job["nxx"]), int(job["nzz"])
+    N1, Nq = nxx * nzz, (nxx // 2 + 1) * (nzz // 2 + 1)
+    g = np.ascontiguousarray(job["gam"], np.float32).ravel()
+    w = np.ascontiguousarray(job["vwd"], np.float32).ravel()
+    cc = lowrank_cc(job["vp"], job["lam"], job["dt"]).ravel()
+    U = np.zeros(N1 * rank)
+    A = np.zeros(rank * rank)
+    V = np.zeros(rank * Nq)
+    rc = lib().cqh_lowrank(C.c_int(nxx), C.c_int(nzz), C.c_float(job["dx"]), C.c_float(job["dz"]), _fp(g), _fp(w),
+                           _fp(cc), C.c_int(symbol), C.c_int(rank), C.c_ulonglong(seed),
+                           U.ctypes.data_as(_D), A.ctypes.data_as(_D), V.ctypes.data_as(_D))
+    if rc != 0:
+        raise ValueError("low-rank decomposition failed")
+    return U.reshape(N1, rank), A.reshape(rank, rank), V.reshape(rank, Nq)
+
+
+def lowrank_pack(job, U, A, V):
+    """Engine inputs (weights [r, N1], symbols / N [r, Nq]) as float32."""
+    nxx, nzz = int(job["nxx"]), int(job["nzz"])
+    r = A.shape[0]
+    N1, Nq = nxx * nzz, (nxx // 2 + 1) * (nzz // 2 + 1)
+    wts = np.zeros(r * N1, np.float32)
+    sym = np.zeros(r * Nq, np.float32)
+    lib().cqh_lowrank_pack(C.c_int(nxx), C.c_int(nzz), C.c_int(r), np.ascontiguousarray(U).ctypes.data_as(_D),
+                           np.ascontiguousarray(A).ctypes.data_as(_D), np.ascontiguousarray(V).ctypes.data_as(_D),
+                           _fp(wts), _fp(sym))
+    return wts.reshape(r, N1), sym.reshape(r, Nq)
+
+
+def lowrank_error(job, U, A, V, cells, symbol=LR_FRACLAP):
+    """checka_error: max |W - U A V| / max |W| over the given cells."""
+    nxx, nzz = int(job["nxx"]), int(job["nzz"])
+    g = np.ascontiguousarray(job["gam"], np.float32).ravel()
+    w = np.ascontiguousarray(job["vwd"], np.float32).ravel()
+    cc = lowrank_cc(job["vp"], job["lam"], job["dt"]).ravel()
+    cells = np.ascontiguousarray(cells, np.int32)
+    return float(lib().cqh_lowrank_error(C.c_int(nxx), C.c_int(nzz), C.c_float(job["dx"]), C.c_float(job["dz"]),
+                                         _fp(g), _fp(w), _fp(cc), C.c_int(symbol), C.c_int(A.shape[0]),
+                                         np.ascontiguousarray(U).ctypes.data_as(_D),
+                                         np.ascontiguousarray(A).ctypes.data_as(_D),
+                                         np.ascontiguousarray(V).ctypes.data_as(_D), C.c_int(cells.size), _ip(cells)))
+
+
+def lowrank_kgrid(nxx, nzz, dx, dz):
+    k2 = np.zeros((nxx // 2 + 1) * (nzz // 2 + 1))
+    lib().cqh_lr_kgrid(C.c_int(nxx), C.c_int(nzz), C.c_float(dx), C.c_float(dz), k2.ctypes.data_as(_D))
+    return k2

@@ -298,3 +298,15 @@ def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402, wide
         job["etas"] = job["etar"] = 1e-20
         jobs.append(job)
     return jobs
+
+
+def with_lowrank(job, rank=4, symbol=None, seed=1):
+    """The job with its viscoacoustic operator in rank-r low-rank form (FP64
+    host decomposition, cqh_lowrank; default symbol: the fractional-Laplacian
+    operator).  Returns (job, (U, A, V)); the decomposition feeds the
+    oracle's low-rank apply."""
+    sym = host.LR_FRACLAP if symbol is None else symbol
+    U, A, V = host.lowrank(job, rank, symbol=sym, seed=seed)
+    w, s = host.lowrank_pack(job, U, A, V)
+    out = dict(job, lr_rank=rank, lr_w=w, lr_sym=s)
+    return out, (U, A, V)

@@ -468,10 +468,13 @@ static const size_t kLdsMax = 160 * 1024;
 #endif
 static const size_t kLdsTile = CQ_LDS_TILE;
 
-enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_QC2, P_QR2, P_QC3, P_QR3, P_COUNT };
+// In launch order: a job's passes run in this order within a step (low-rank
+// jobs take LR1 / LR2 where the others take QR1 / QC2).
+enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_LR1, P_QC2, P_LR2, P_QR2, P_QC3, P_QR3, P_COUNT };
+static_assert(P_COUNT == CQ_NPASS, "pass count of the C ABI");
 static const int kGroups = 4;   // launch groups of a batch (see cq_batch_s::goff)
-static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1",
-                                         "k_q_c2",   "k_q_r2",   "k_q_c3", "k_q_r3"};
+static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1", "k_lr_r1",
+                                         "k_q_c2",   "k_lr_c2",  "k_q_r2", "k_q_c3", "k_q_r3"};
 
 // A set of shots advanced together: every launch of a pass covers the tiles
 // of every wavefield in the set, and each wavefield decides from its step
@@ -603,6 +606,9 @@ static int validate(const cq_job_desc &j, int idx)
         if (j.gxg[i] < 0 || j.gxg[i] >= j.nxx || j.gzg[i] < 0 || j.gzg[i] >= j.nzz)
             return set_err(CQ_EINVAL, "job %d: receiver %d outside the window", idx, i);
     if (!(j.dx > 0.f) || !(j.dz > 0.f)) return set_err(CQ_EINVAL, "job %d: bad dx/dz", idx);
+    if (j.lr_rank < 0 || j.lr_rank > 8) return set_err(CQ_EINVAL, "job %d: lr_rank %d outside 0..8", idx, j.lr_rank);
+    if (j.lr_rank > 0 && (!is_q(j.kind) || !j.lr_w || !j.lr_sym))
+        return set_err(CQ_EINVAL, "job %d: a low-rank operator needs a viscoacoustic kind, lr_w and lr_sym", idx);
     if (is_rtm(j.kind)) {
         // revolve limits checkup = 64 (src.c:176) and snaps >= 1
         if (j.snaps < 1 || j.snaps > 64) return set_err(CQ_EINVAL, "job %d: snaps %d outside 1..64", idx, j.snaps);
@@ -708,6 +714,14 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         o_s3 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
         o_s4 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
     }
+    // low-rank operator: weights (field layout), quarter-plane symbols, half spectra
+    const int lr = q ? j.lr_rank : 0, nxh = nxx / 2 + 1;
+    size_t o_lrw = 0, o_lrv = 0, o_lrs = 0;
+    if (lr) {
+        o_lrw = A.take<float>(N * lr);
+        o_lrv = A.take<float>((size_t)nxh * nzh * lr);
+        o_lrs = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk * lr);
+    }
     const size_t o_1x = A.take<float>((size_t)4 * nxx), o_1z = A.take<float>((size_t)4 * nzz);
     const size_t o_mx = A.take<float2>((size_t)3 * nxx), o_mz = A.take<float2>((size_t)3 * nzz);
     const size_t o_rik = A.take<float>(j.nt);
@@ -769,6 +783,13 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         base.tot = (float *)(mem + o_tot);
         base.scr = (float *)(mem + o_scr);
         base.s1 =(float2 *)(mem + o_s1); base.s3 = (float2 *)(mem + o_s3); base.s4 = (float2 *)(mem + o_s4);
+    }
+    base.lr_rank = lr;
+    base.nxh = nxh;
+    if (lr) {
+        base.lrw = (float *)(mem + o_lrw);
+        base.lrv = (float *)(mem + o_lrv);
+        base.lrs = (float2 *)(mem + o_lrs);
     }
     float *x1 = (float *)(mem + o_1x), *z1 = (float *)(mem + o_1z);
     base.iax = x1; base.ibx = x1 + nxx; base.hax = x1 + 2 * nxx; base.hbx = x1 + 3 * nxx;
@@ -894,6 +915,17 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     if (q)
         for (int i = 0; i < 4; i++)
             HIPCHK(hipMemcpyAsync(mem + o_q[i], hq[i].data(), 4 * N, hipMemcpyHostToDevice, s));
+    std::vector<float> hlw;
+    if (lr) {
+        // weights to the field layout; symbols as given (quarter plane)
+        hlw.assign(N * lr, 0.f);
+        for (int i = 0; i < lr; i++)
+            for (int ix = 0; ix < nxx; ix++)
+                for (int iz = 0; iz < nzz; iz++)
+                    hlw[N * i + cq_q4(nxx, ix, iz >> 2) + (iz & 3)] = j.lr_w[(size_t)i * nxx * nzz + (size_t)ix * nzz + iz];
+        HIPCHK(hipMemcpyAsync((void *)base.lrw, hlw.data(), 4 * hlw.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync((void *)base.lrv, j.lr_sym, 4 * (size_t)nxh * nzh * lr, hipMemcpyHostToDevice, s));
+    }
     HIPCHK(hipMemcpyAsync(x1, h1x.data(), 4 * h1x.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(z1, h1z.data(), 4 * h1z.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(mx, hmx.data(), 8 * hmx.size(), hipMemcpyHostToDevice, s));
@@ -971,8 +1003,9 @@ static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out)
         out[P_AROW] = lds_lines(nxx, b->tz / 2);
         out[P_ACOL] = lds_lines(nzz, b->cw / 2);
     } else {
-        out[P_QC1] = out[P_QC2] = out[P_QC3] = lds_lines(nzz, b->cwq / 2);
+        out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = lds_lines(nzz, b->cwq / 2);
         out[P_QR1] = r1_lds(nxx, b->tk);
+        out[P_LR1] = lds_lines(nxx, 2 * b->tk);
         out[P_QR2] = lds_lines(nxx, b->tk2);
         out[P_QR3] = lds_lines(nxx, b->tz / 2);
     }
@@ -1102,10 +1135,11 @@ static int rebuild_tables(cq_batch b)
             } else {
                 for (int x0 = 0; x0 < h.nxx; x0 += b->cwq) {
                     it[P_QC1].push_back(make_int2(e, x0));
-                    it[P_QC2].push_back(make_int2(e, x0));
+                    it[h.d[0].lr_rank > 0 ? P_LR2 : P_QC2].push_back(make_int2(e, x0));
                     it[P_QC3].push_back(make_int2(e, x0));
                 }
-                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) it[P_QR1].push_back(make_int2(e, k0));
+                const bool lr = h.d[0].lr_rank > 0;
+                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) it[lr ? P_LR1 : P_QR1].push_back(make_int2(e, k0));
                 for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk2) it[P_QR2].push_back(make_int2(e, k0));
                 for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_QR3].push_back(make_int2(e, z0));
             }
@@ -1238,7 +1272,12 @@ static int launch_pass(cq_batch b, int pass, int role, int step)
         return b->tk == 8   ? launch(b, k_q_r1<8>, CQ_ARGS)
                : b->tk == 4 ? launch(b, k_q_r1<4>, CQ_ARGS)
                : b->tk == 2 ? launch(b, k_q_r1<2>, CQ_ARGS) : launch(b, k_q_r1<1>, CQ_ARGS);
+    case P_LR1:
+        return b->tk == 8   ? launch(b, k_lr_r1<8>, CQ_ARGS)
+               : b->tk == 4 ? launch(b, k_lr_r1<4>, CQ_ARGS)
+               : b->tk == 2 ? launch(b, k_lr_r1<2>, CQ_ARGS) : launch(b, k_lr_r1<1>, CQ_ARGS);
     case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
+    case P_LR2: return CQ_DISPATCH_CW(k_lr_c2, b->cwq);
     case P_QR2: return CQ_DISPATCH_TK(k_q_r2, b->tk2);
     case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
     case P_QR3: return CQ_DISPATCH_TZ(k_q_r3);
@@ -1601,6 +1640,10 @@ static void copy_job(ServerJob &sj, const cq_job_desc &j)
     own(sj.d.kxsmr, j.nxx); own(sj.d.kxsmi, j.nxx); own(sj.d.kxscr, j.nxx); own(sj.d.kxsci, j.nxx);
     own(sj.d.kzsmr, j.nzz); own(sj.d.kzsmi, j.nzz); own(sj.d.kzscr, j.nzz); own(sj.d.kzsci, j.nzz);
     own(sj.d.rik, (size_t)j.nt);
+    if (is_q(j.kind) && j.lr_rank > 0) {
+        own(sj.d.lr_w, (size_t)j.lr_rank * nc);
+        own(sj.d.lr_sym, (size_t)j.lr_rank * (j.nxx / 2 + 1) * (j.nzz / 2 + 1));
+    }
     if (is_rtm(j.kind) && j.nr > 0) own(sj.d.obs, (size_t)j.nt * j.nr);
     else sj.d.obs = nullptr;
     if (j.nr > 0) {

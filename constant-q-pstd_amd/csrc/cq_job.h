@@ -73,7 +73,12 @@ __host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
     float G *img; float G *ssg;                                                                   \
     const float G *psrc;                                                                          \
     const float G *obs;                                                                           \
-    const int G *inj_x; const int G *inj_z; const int G *inj_i; const int G *inj_off;
+    const int G *inj_x; const int G *inj_z; const int G *inj_i; const int G *inj_off;             \
+    /* rank-r low-rank operator (lr_rank > 0): weights lrw[i] (field layout, nxx ldz each), */    \
+    /* symbols / N lrv[i] on the quarter plane (nxh x nzh, kz fastest), and the rank's    */    \
+    /* half spectra lrs[i] (half-spectrum layout, cq_hs_cols(nxx) ldk each)                */    \
+    int lr_rank, nxh;                                                                             \
+    const float G *lrw; const float G *lrv; float2 G *lrs;
 
 struct JobD {
     CQ_JOB_FIELDS()

@@ -919,6 +919,121 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_r1(const JobD *_
     krow_store<TK>(buf, J, J.s2, k0);
 }
 
+// ===========================================================================
+// low-rank form of the viscoacoustic operator (jobs with lr_rank > 0):
+//   total = sum_i w_i(x) F^-1[s_i(k) F div]
+// in place of sum_total (the apply of sgpstd-qmodeling/acps4.cu:1337-1459:
+// multv_kernel, then sumlw / sumlw2 with the A factor folded into w_i).
+//   LR1 (kz rows) : aux as R1; F_x div once, then per rank x s_i, F_x^-1 -> lrs[i]
+//   LR2 (columns) : per rank F_z^-1 lrs[i], total += w_i x; F_z total -> s1
+// ===========================================================================
+// Line dst + l (of TK) = spectrum line l x s_i(|kx|, kz) (symbols already
+// divided by N); padding kz rows (kz >= nzh) get zero.
+template <int TK>
+__device__ __forceinline__ void krow_lr_mult(const JobG &J, float2 *buf, int dst, int k0, int i)
+{
+    const int nxx = J.nxx, LS = J.lsx;
+    const float inv = __builtin_amdgcn_rcpf((float)nxx);
+    const float CQ_GLOBAL *v = J.lrv + (size_t)i * J.nxh * J.nzh;
+    for (int e = threadIdx.x; e < nxx * TK; e += CQ_NT) {
+        const int l = cq_div(e, inv), kx = e - l * nxx, pe = l * LS + kx, kz = k0 + l;
+        const int ikx = kx <= nxx / 2 ? kx : nxx - kx;
+        const float m = kz < J.nzh ? v[(size_t)ikx * J.nzh + kz] : 0.f;
+        const float2 w = buf[pe];
+        buf[dst * LS + pe] = make_float2(w.x * m, w.y * m);
+    }
+}
+
+template <int TK>
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_lr_r1(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                 int step)
+{
+    CQ_ITEM_PROLOGUE
+    const int k0 = it.y, LS = J.lsx;
+    float2 *const lds = cq_lds;
+    float2 *tw = lds, *buf = lds + J.twlx;
+    load_tw(tw, J.twx, J.twlx);
+    // aux = |k|^1.01 p of the previous step (stabilised for Q-RTM), as k_q_r1
+    krow_fill<TK>(buf, J, J.s2, k0);
+    __syncthreads();
+    cq_fft_fwd(Jr.px, buf, LS, TK, tw, nullptr);
+    krow_mult<TK>(J, buf, k0, J.stab ? 4 : 3, (float)t_aux * J.dt);
+    __syncthreads();
+    cq_fft_inv(Jr.px, buf, LS, TK, tw);
+    krow_store<TK>(buf, J, J.s2, k0);
+    __syncthreads();
+    // the div spectrum stays in lines 0..TK; each rank's product goes
+    // through lines TK..2 TK
+    krow_fill<TK>(buf, J, J.s1, k0);
+    __syncthreads();
+    cq_fft_fwd(Jr.px, buf, LS, TK, tw, nullptr);
+    const size_t hsz = (size_t)cq_hs_cols(J.nxx) * J.ldk;
+    for (int i = 0; i < J.lr_rank; i++) {
+        krow_lr_mult<TK>(J, buf, TK, k0, i);
+        __syncthreads();
+        cq_fft_inv(Jr.px, buf + TK * LS, LS, TK, tw);
+        krow_store<TK>(buf + TK * LS, J, J.lrs + hsz * i, k0);
+        __syncthreads();
+    }
+}
+
+template <int CW>
+__global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_lr_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
+                                                 int step)
+{
+    constexpr int L = CW / 2;
+    CQ_ITEM_PROLOGUE
+    const int nzz = J.nzz, x0 = it.y, nq = (nzz + 3) >> 2;
+    float2 *const lds = cq_lds;
+    float2 *tw = lds, *buf = lds + J.twlz;
+    float *bufs = reinterpret_cast<float *>(buf);
+    load_tw(tw, J.twz, J.twlz);
+    const size_t hsz = (size_t)cq_hs_cols(J.nxx) * J.ldk, fsz = (size_t)J.nxx * J.ldz;
+    const int r = J.lr_rank;
+    for (int i = 0; i < r; i++) {
+        col_insert<CW>(J, buf, x0, J.lrs + hsz * i);
+        __syncthreads();
+        cq_fft_inv(Jr.pz, buf, J.lsz, L, tw);
+        // total accumulates in the field `tot`; each thread re-reads only the
+        // quads it wrote; the last rank leaves total in the lines as well
+        const bool first = i == 0, last = i == r - 1;
+        cq_loop<2, true>(
+            CW * nq,
+            [&](int e) {
+                const int zq = e / CW, c = e - zq * CW;
+                const size_t g = cq_q4(J.nxx, min(x0 + c, J.nxx - 1), zq);
+                F4x2 v;
+                v.a = ld4(J.lrw + fsz * i + g);
+                v.b = first ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4c(J.tot + g);
+                return v;
+            },
+            [&](int e, const F4x2 &v) {
+                const int zq = e / CW, c = e - zq * CW, ix = x0 + c;
+                float *line = col_line(bufs, J.lsz, c);
+                if (ix >= J.nxx) {
+                    if (last)
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (4 * zq + k < nzz) line[2 * (4 * zq + k)] = 0.f;
+                    return;
+                }
+                float4 acc = v.b;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int iz = 4 * zq + k;
+                    if (iz >= nzz) continue;
+                    const float tv = f4get(acc, k) + f4get(v.a, k) * line[2 * iz];
+                    f4set(acc, k, tv);
+                    if (last) line[2 * iz] = tv;
+                }
+                st4c(J.tot + cq_q4(J.nxx, ix, zq), acc);
+            });
+        __syncthreads();
+    }
+    cq_fft_fwd(Jr.pz, buf, J.lsz, L, tw, nullptr);
+    col_extract<CW>(J, buf, x0, J.s1, nullptr);
+}
+
 template <int CW>
 __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c2(const JobD *__restrict__ jobs, const int2 *__restrict__ items,
                                                 int step)

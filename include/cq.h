@@ -70,6 +70,16 @@ typedef struct {
     int snaps;                /* revolve checkpoints, 1..64 (input file "snapnum") */
     float etas, etar;         /* Q-RTM stabilisers of the source / receiver
                                * wavefields, in units of 1e-6 (visac2drtm.txt) */
+    /* Kinds VISCO / QRTM: the operator applied to the velocity divergence,
+     * total = sum_i w_i(x) F^-1[s_i(k) F div], as a rank-r low-rank form
+     * (cq_lowrank.h: cqh_lowrank + cqh_lowrank_pack, the FP64 host
+     * decomposition of sgpstd-qmodeling/sglowrank.c:148-480 applied like
+     * acps4.cu:1337-1459).  lr_rank 0 keeps the reference's live Taylor form
+     * (sum_total, acsglw.cu:1213-1245). */
+    int lr_rank;              /* 0, or 1..8 */
+    const float *lr_w;        /* lr_rank * nxx*nzz per-cell weights, x-major */
+    const float *lr_sym;      /* lr_rank * (nxx/2+1)*(nzz/2+1) symbols / (nxx nzz) on the
+                               * quarter plane |kx|, |kz| (kz fastest) */
 } cq_job_desc;
 
 /* Opens device `device` (HIP ordinal) and creates the engine's stream. */
@@ -185,10 +195,12 @@ int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec);
 
 /* Per-pass device timing with HIP events on the engine's stream.  While
  * enabled, every launch of cq_batch_step is bracketed by events; passes are
- * numbered 0..7 (cq_pass_name gives the kernel name): acoustic row, acoustic
- * column, then the six viscoacoustic passes C1 R1 C2 R2 C3 R3. */
+ * numbered 0..CQ_NPASS-1 (cq_pass_name gives the kernel name): acoustic row,
+ * acoustic column, then the viscoacoustic passes C1 R1 LR1 C2 LR2 R2 C3 R3
+ * (LR1 / LR2 replace R1 / C2 for low-rank jobs). */
+#define CQ_NPASS 10
 int cq_batch_enable_timing(cq_batch b, int on);
-/* Average ms per launch of each pass (arrays of 8) and launch counts. */
+/* Average ms per launch of each pass (arrays of CQ_NPASS) and launch counts. */
 int cq_batch_kernel_times(cq_batch b, double *ms, int *calls);
 const char *cq_pass_name(int pass);
 /* Acoustic row / column averages (passes 0 and 1). */

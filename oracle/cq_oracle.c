@@ -420,6 +420,56 @@ static void p_update_ac(const orc_model *m, orc_state *s, orc_work *w)
     }
 }
 
+/* The rank-r low-rank operator (set by orc_set_lowrank; rank 0: none).  U is
+ * N1 x r (U[x * r + j]), A is r x r (A[j * r + i]), V is r x Nq on the
+ * quarter plane (V[i * Nq + ikx * nzh + ikz]), as cqh_lowrank returns them. */
+static struct {
+    int rank;
+    const real *U, *A, *V;
+} g_lr;
+
+void orc_set_lowrank(int rank, const real *U, const real *A, const real *V)
+{
+    g_lr.rank = rank;
+    g_lr.U = U;
+    g_lr.A = A;
+    g_lr.V = V;
+}
+
+/* total = sum_j U(x, j) sum_i A(j, i) Re IFFT(V_i FFT div / N): the apply
+ * kernels of the reference's low-rank path (sgpstd-qmodeling/acps4.cu:
+ * 1337-1459): multv_kernel (V mirrored to the full grid as consist_Vt,
+ * sglowrank.c:104-145), the inverse transform, fuzhi_r_kernel + sumlw_kernel
+ * (accumulators in double) and sumlw2_kernel.  fdiv holds FFT(div). */
+static void lowrank_total(const orc_model *m, orc_work *w, const cpx *fdiv)
+{
+    const int nxx = m->nxx, nzz = m->nzz, r = g_lr.rank, nzh = nzz / 2 + 1;
+    const size_t N = (size_t)nxx * nzz, Nq = (size_t)(nxx / 2 + 1) * nzh;
+    const real inv_n = (real)1 / (real)(double)N;
+    double *pst = (double *)calloc(N * (size_t)r, sizeof(double));
+    for (int i = 0; i < r; i++) {
+        for (int ix = 0; ix < nxx; ix++)
+            for (int iz = 0; iz < nzz; iz++) {
+                const size_t a = (size_t)ix * nzz + iz;
+                const int ikx = ix <= nxx / 2 ? ix : nxx - ix, ikz = iz <= nzz / 2 ? iz : nzz - iz;
+                const real v = g_lr.V[(size_t)i * Nq + (size_t)ikx * nzh + ikz];
+                w->d2[a].re = fdiv[a].re * v * inv_n;
+                w->d2[a].im = fdiv[a].im * v * inv_n;
+            }
+        fft2d(&w->px, &w->pz, w->d2, nxx, nzz, 1);
+        for (int j = 0; j < r; j++) {
+            const double aji = (double)g_lr.A[(size_t)j * r + i];
+            for (size_t a = 0; a < N; a++) pst[(size_t)j * N + a] += aji * (double)w->d2[a].re;
+        }
+    }
+    for (size_t a = 0; a < N; a++) {
+        double t = 0.0;
+        for (int j = 0; j < r; j++) t += (double)g_lr.U[a * r + j] * pst[(size_t)j * N + a];
+        w->total[a] = (real)t;
+    }
+    free(pst);
+}
+
 /* Viscoacoustic: sum_total (acsglw.cu:1213-1245), FFT(total)*k2da
  * (:578-588) and the p update (addsa_kernel, :1500-1537).  sgn = +1 for
  * modeling; -1 flips the tao terms for Q-compensated back-propagation
@@ -432,6 +482,9 @@ static void p_update_q(const orc_model *m, orc_state *s, orc_work *w, real sgn)
     /* d1 = FFT(div) */
     memcpy(w->d1, w->d0, sizeof(cpx) * N);
     fft2d(&w->px, &w->pz, w->d1, nxx, nzz, 0);
+    if (g_lr.rank > 0) {
+        lowrank_total(m, w, w->d1);
+    } else {
     mult_k_ifft(m, w, w->d1, m->k2d2, w->d2);
     mult_k_ifft(m, w, w->d1, m->k2d1, w->d3);
     mult_k_ifft(m, w, w->d1, m->k2d3, w->d4);
@@ -445,6 +498,7 @@ static void p_update_q(const orc_model *m, orc_state *s, orc_work *w, real sgn)
         real dtf = dt * dt;
         w->total[a] = y1 * w->d0[a].re + y2 * w->d2[a].re -
                       (real)(1.0 / 12.0) * dtf * vpf * m->lam[a] * (y3 * w->d3[a].re + y4 * w->d4[a].re);
+    }
     }
     for (size_t a = 0; a < N; a++) { w->d1[a].re = w->total[a]; w->d1[a].im = 0; }
     fft2d(&w->px, &w->pz, w->d1, nxx, nzz, 0);

@@ -85,7 +85,27 @@ class _Args:
         self.gzg = np.ascontiguousarray(job["gzg"], np.int32)
 
 
-def forward(job, bits=64, want_state=False, init=None):
+class lowrank_set:
+    """Context: the oracle's viscoacoustic operator in rank-r low-rank form
+    (U, A, V of cqh_lowrank) for the calls inside, as acps4.cu:1337-1459
+    applies it; none (the Taylor form) otherwise."""
+
+    def __init__(self, lr, bits=64):
+        self.bits = bits
+        dt = _dt(bits)
+        self.arrs = None if lr is None else [np.ascontiguousarray(a, dt).ravel() for a in lr]
+        self.rank = 0 if lr is None else int(np.asarray(lr[1]).shape[0])
+
+    def __enter__(self):
+        if self.arrs is not None:
+            lib(self.bits).orc_set_lowrank(C.c_int(self.rank), *[_ptr(a, self.bits) for a in self.arrs])
+        return self
+
+    def __exit__(self, *a):
+        lib(self.bits).orc_set_lowrank(C.c_int(0), None, None, None)
+
+
+def forward(job, bits=64, want_state=False, init=None, lowrank=None):
     """One-shot forward modeling on the CPU.  job: dict with kind (0 acoustic,
     1 viscoacoustic), nxx, nzz, nt, dt and the float32 arrays of the job
     (window fields, 1-D coefficients, k powers, rik, gxg, gzg, xsg, zsg).
@@ -109,19 +129,20 @@ def forward(job, bits=64, want_state=False, init=None):
         else:
             ic.append(None)
     icp = [None if v is None else _ptr(v, bits) for v in ic]
-    L.orc_forward_ic(C.c_int(int(job["kind"])), C.c_int(int(job["nxx"])), C.c_int(int(job["nzz"])),
-                     C.c_int(nt), real(float(job["dt"])), a.fields, a.c1d, a.k2, _ptr(a.rik, bits),
-                     C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])), C.c_int(nr),
-                     a.gxg.ctypes.data_as(C.POINTER(C.c_int)),
-                     a.gzg.ctypes.data_as(C.POINTER(C.c_int)), _ptr(rec, bits), _ptr(p, bits),
-                     _ptr(vx, bits), _ptr(vz, bits), icp[0], icp[1], icp[2])
+    with lowrank_set(lowrank, bits):
+        L.orc_forward_ic(C.c_int(int(job["kind"])), C.c_int(int(job["nxx"])), C.c_int(int(job["nzz"])),
+                         C.c_int(nt), real(float(job["dt"])), a.fields, a.c1d, a.k2, _ptr(a.rik, bits),
+                         C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])), C.c_int(nr),
+                         a.gxg.ctypes.data_as(C.POINTER(C.c_int)),
+                         a.gzg.ctypes.data_as(C.POINTER(C.c_int)), _ptr(rec, bits), _ptr(p, bits),
+                         _ptr(vx, bits), _ptr(vz, bits), icp[0], icp[1], icp[2])
     rec = rec.reshape(nt, nr)
     if want_state:
         return rec, {"p": p, "vx": vx, "vz": vz}
     return rec
 
 
-def rtm(job, trace, bits=64):
+def rtm(job, trace, bits=64, lowrank=None):
     """RTM / Q-RTM of one shot (job kind 2 or 3, with obs, snaps, etas, etar)
     following the revolve action trace [(action, capo, check, fine), ...].
     Returns the raw interior (img, ssg), each (nx, nz)."""
@@ -138,6 +159,12 @@ def rtm(job, trace, bits=64):
     ssg = np.zeros(nx * nz, dt_)
     real = C.c_double if bits == 64 else C.c_float
     ip = lambda v: v.ctypes.data_as(C.POINTER(C.c_int))
+    with lowrank_set(lowrank, bits):
+        _rtm_call(L, job, a, bits, nxx, nzz, pml, obs, act, capo, check, img, ssg, real, ip)
+    return img.reshape(nx, nz), ssg.reshape(nx, nz)
+
+
+def _rtm_call(L, job, a, bits, nxx, nzz, pml, obs, act, capo, check, img, ssg, real, ip):
     L.orc_rtm(C.c_int(int(job["kind"])), C.c_int(nxx), C.c_int(nzz), C.c_int(pml),
               C.c_int(int(job["nt"])), real(float(job["dt"])), a.fields, a.c1d, a.k2,
               _ptr(a.rik, bits), C.c_int(int(job["xsg"])), C.c_int(int(job["zsg"])),
@@ -145,7 +172,6 @@ def rtm(job, trace, bits=64):
               ip(check), C.c_int(len(act)), C.c_int(int(job["snaps"])),
               real(float(job.get("etas", 0.0))), real(float(job.get("etar", 0.0))),
               _ptr(img, bits), _ptr(ssg, bits))
-    return img.reshape(nx, nz), ssg.reshape(nx, nz)
 
 
 def set_threads(n, bits=64):

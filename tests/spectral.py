@@ -59,7 +59,7 @@ def point_spectrum(nxx, nzz, ix, iz, amp=1.0):
 
 
 def simulate(nxx, nzz, dx, dz, dt, nt, vp, rho, rik=None, src=None, recs=(), kind=0,
-             cpml=None, init=None, q=None, qsign=1.0, stab=None, replace=None, t_index=None):
+             cpml=None, init=None, q=None, qsign=1.0, stab=None, replace=None, t_index=None, W=None):
     """Advances the homogeneous periodic medium nt steps.
 
     kind 0 acoustic (acps4.cu:508-607), 1 viscoacoustic (acsglw.cu:511-645).
@@ -67,6 +67,8 @@ def simulate(nxx, nzz, dx, dz, dt, nt, vp, rho, rik=None, src=None, recs=(), kin
     axes; None for no absorption.  init: dict of initial real fields p, vx, vz.
     q: dict of scalar gam, lam, vwd, tao.  qsign -1 flips the loss terms
     (Q-compensated back-propagation, sgpstd-qrtm/acsglw.cu:1769-1806).
+    W: optional function of k2 replacing the Taylor operator applied to div
+    (the low-rank path's symbol).
     stab: (eta, taoc) for the stabilised aux multiplier of Q-RTM, with the
     step's time index from t_index(t) (default t).  replace: (cells, values)
     with values[t][j] written into p at cells[j] after the p update (RTM
@@ -94,8 +96,11 @@ def simulate(nxx, nzz, dx, dz, dt, nt, vp, rho, rik=None, src=None, recs=(), kin
         s2 = k2 ** (1.0 / 32.0)                     # |k|^(1/16)
         ka = k2 ** 0.505                            # |k|^1.01
         # the mixed-domain operator applied to div (sum_total, acsglw.cu:1213-1245)
-        W = (1 - 32 * g) + 32 * g * vwd * s2 - dt * dt * vp * vp * lam / 12.0 * (
-            (1 - 64 * g) * k2 + 64 * g * vwd * k2 * s2)
+        if W is not None:
+            W = W(k2)
+        else:
+            W = (1 - 32 * g) + 32 * g * vwd * s2 - dt * dt * vp * vp * lam / 12.0 * (
+                (1 - 64 * g) * k2 + 64 * g * vwd * k2 * s2)
     for t in range(nt):
         dxv, dzv = hxm * X, hzm * Z
         if cpml is not None:
