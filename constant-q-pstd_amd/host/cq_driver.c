@@ -25,6 +25,7 @@
 
 #include "cq.h"
 #include "cq_host.h"
+#include "cq_lowrank.h"
 
 #define DIE(...)                                   \
     do {                                           \
@@ -42,6 +43,7 @@ static double now_s(void)
 
 typedef struct {
     float *vp, *rho, *bbx, *bbz, *gam, *lam, *vwd, *tao;
+    float *lrw, *lrs;          /* low-rank weights and symbols (vaps / qrtm with "low-rank") */
     int *gxg, *gzg;
     float *obs;
     cqh_shot_geom geo;
@@ -62,6 +64,7 @@ static void free_shot(shot_buf *s)
     free(s->vp); free(s->rho); free(s->bbx); free(s->bbz);
     free(s->gam); free(s->lam); free(s->vwd); free(s->tao);
     free(s->gxg); free(s->gzg); free(s->obs);
+    free(s->lrw); free(s->lrs);
     memset(s, 0, sizeof(*s));
 }
 
@@ -217,6 +220,25 @@ int main(int argc, char **argv)
                 j->xsg = b->geo.xsg; j->zsg = b->geo.zsg;
                 j->nr = b->nr; j->gxg = b->gxg; j->gzg = b->gzg;
                 j->obs = b->obs; j->snaps = P.snapnum; j->etas = P.etas; j->etar = P.etar;
+                if (q && P.lr_rank > 0) {
+                    /* the viscoacoustic operator in low-rank form: FP64
+                     * decomposition of this shot's window, seeded by its id */
+                    const int r = P.lr_rank;
+                    const size_t N1 = (size_t)nxx * nzz, Nq = (size_t)(nxx / 2 + 1) * (nzz / 2 + 1);
+                    float *cc = (float *)malloc(sizeof(float) * N1);
+                    const float dtf = dt * dt, c12 = (float)(1.0 / 12.0);
+                    for (size_t a = 0; a < N1; a++) cc[a] = c12 * dtf * (b->vp[a] * b->vp[a]) * b->lam[a];
+                    double *U = (double *)malloc(sizeof(double) * N1 * r), *A = (double *)malloc(sizeof(double) * r * r),
+                           *V = (double *)malloc(sizeof(double) * Nq * r);
+                    if (cqh_lowrank(nxx, nzz, dx, dz, b->gam, b->vwd, cc, CQH_LR_FRACLAP, r,
+                                    1ULL + (unsigned long long)sid, U, A, V))
+                        DIE("low-rank decomposition of shot %d failed", sid);
+                    b->lrw = (float *)malloc(sizeof(float) * N1 * r);
+                    b->lrs = (float *)malloc(sizeof(float) * Nq * r);
+                    cqh_lowrank_pack(nxx, nzz, r, U, A, V, b->lrw, b->lrs);
+                    free(U); free(A); free(V); free(cc);
+                    j->lr_rank = r; j->lr_w = b->lrw; j->lr_sym = b->lrs;
+                }
                 printf("shotid=%d nxx=%d nzz=%d xs=%f zs=%f\n", sid, nxx, nzz, xs, zs);
             }
             cq_batch bt;

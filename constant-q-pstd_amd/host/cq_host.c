@@ -80,13 +80,14 @@ int cqh_read_params(const char *path, int fmt, cqh_params *o)
     e |= rd_int(fp, &o->flagsource);
     if (rtm) e |= rd_int(fp, &o->snapnum);
     if (e) { fclose(fp); return -2; }
-    /* optional trailing "label / value" pairs: only "pml" is understood */
+    /* optional trailing "label / value" pairs: "pml" and "low-rank" */
     {
         char lab[256];
         while (fscanf(fp, " ") >= 0 && fgets(lab, sizeof lab, fp)) {
             int v;
             if (fscanf(fp, " %d", &v) != 1) break;
-            if (strstr(lab, "pml") || strstr(lab, "PML")) o->pml = v;
+            if (strstr(lab, "low-rank") || strstr(lab, "lowrank")) o->lr_rank = v;
+            else if (strstr(lab, "pml") || strstr(lab, "PML")) o->pml = v;
         }
     }
     fclose(fp);

@@ -26,7 +26,8 @@ extern "C" {
  *   visac2drtm.txt       (sgpstd-qrtm/func.c:9-146)
  * Optional trailing label/value pairs understood after the reference's last
  * value: "pml" (CPML width, default 10 = the reference's #define) and
- * "snaps" handled by the RTM formats already. */
+ * "low-rank" (vaps / qrtm: rank of the low-rank form of the viscoacoustic
+ * operator, cq_lowrank.h; default 0 = the reference's Taylor form). */
 typedef struct {
     int shotnum, tmax;
     float dt_ms, fd, fo;
@@ -38,6 +39,7 @@ typedef struct {
     char recname[60];
     int flagout, flagsource, snapnum;
     int pml;                       /* optional trailing key, default 10 */
+    int lr_rank;                   /* optional trailing key, default 0 */
 } cqh_params;
 
 enum { CQH_FMT_APS = 0, CQH_FMT_VAPS = 1, CQH_FMT_ARTM = 2, CQH_FMT_QRTM = 3 };

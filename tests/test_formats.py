@@ -26,13 +26,25 @@ class Params(C.Structure):
                 ("fname3", C.c_char * 40), ("fname4", C.c_char * 40), ("fname5", C.c_char * 40),
                 ("vpname", C.c_char * 80), ("rhoname", C.c_char * 80), ("qname", C.c_char * 80),
                 ("recname", C.c_char * 60), ("flagout", C.c_int), ("flagsource", C.c_int),
-                ("snapnum", C.c_int), ("pml", C.c_int)]
+                ("snapnum", C.c_int), ("pml", C.c_int), ("lr_rank", C.c_int)]
 
 
 def read_params(path, fmt):
     p = Params()
     rc = host.lib().cqh_read_params(path.encode(), C.c_int(fmt), C.byref(p))
     return rc, p
+
+
+def test_optional_trailing_keys(tmp_path):
+    """Keys appended after the reference's last value: CPML width and the
+    low-rank rank; absent, they keep the reference's behaviour."""
+    src = open(os.path.join(FIXT, "input", "visac2drealmodeling.txt")).read().rstrip("\n")
+    f = tmp_path / "v.txt"
+    f.write_text(src + "\n--CPML width (grid points)\n  30\n--low-rank operator rank (0: Taylor form)\n  4\n")
+    rc, p = read_params(str(f), 1)
+    assert rc == 0 and (p.pml, p.lr_rank) == (30, 4)
+    rc, p = read_params(os.path.join(FIXT, "input", "visac2drealmodeling.txt"), 1)
+    assert rc == 0 and (p.pml, p.lr_rank) == (10, 0)
 
 
 def test_reference_parameter_files():

@@ -58,6 +58,10 @@ def tree(tmp_path_factory):
                head + grid + geo + ["../output/vp.bin", "../output/rho.bin", 1, 2])
     param_file(t / "input" / "vaps.txt", "visac2drealmodeling.txt",
                head + [FO] + grid + geo + ["../output/vp.bin", "../output/rho.bin", "../output/q.bin", 1, 2])
+    param_file(t / "input" / "vaps_lr.txt", "visac2drealmodeling.txt",
+               head + [FO] + grid + geo + ["../output/vp.bin", "../output/rho.bin", "../output/q.bin", 0, 2])
+    with open(t / "input" / "vaps_lr.txt", "a") as f:
+        f.write("--low-rank operator rank (0: Taylor form)\n  4\n")
     param_file(t / "input" / "artm.txt", "ac2drtm.txt",
                head + grid + [1500.0] + geo + ["../output/vp.bin", "../output/rho.bin", "../obsdata/reca", 2, 12])
     param_file(t / "input" / "qrtm.txt", "visac2drtm.txt",
@@ -133,6 +137,28 @@ def expected_stack(t, kind, vp, rho, q, snaps, v0, eta):
         img_sp += host.projectomodel(imgf, NXT, NZT, nx, nz, PML, job["mlx"], DX)
         ssg_sp += host.projectomodel(ssg.astype(np.float32).ravel(), NXT, NZT, nx, nz, PML, job["mlx"], DX)
     return img_sp, ssg_sp
+
+
+def test_vaps_low_rank_key(tree):
+    """The optional trailing key "low-rank 4" makes vaps decompose each shot's
+    operator (fractional-Laplacian form, seed 1 + shot id) and apply it at
+    rank 4: gathers match the oracle's low-rank apply of the same
+    decomposition (rel-L2 <= 1e-4)."""
+    t, vp, rho, q = tree
+    out = t / "obsdata_lr"
+    out.mkdir(exist_ok=True)
+    print("driver vaps low-rank", flush=True)
+    r = subprocess.run([os.path.join(BIN, "vaps"), "../input/vaps_lr.txt", str(out), "2"], cwd=t / "work",
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nr = 2 * HALF + 1
+    for i, s in enumerate(SHOTS):
+        got = np.fromfile(out / ("recva%d_%d.bin" % (i, nr)), np.float32).reshape(NT, nr)
+        job = shot_job(1, vp, rho, q, s)
+        lr = host.lowrank(job, 4, symbol=host.LR_FRACLAP, seed=1 + i)
+        e = engine.rel_l2(got, orc.forward(job, lowrank=lr))
+        print("vaps low-rank shot", i, "rel-L2 vs FP64 oracle", e)
+        assert e <= 1e-4
 
 
 @pytest.mark.parametrize("mode,out,kind", [("artm", "RTM.BIN", 2), ("qrtm", "QRTM.BIN", 3)])
