@@ -76,21 +76,99 @@ def algorithmic_bytes(jobs):
     return out
 
 
-def cpu_baseline(job, steps, threads):
-    """FP32 reference-faithful CPU path (oracle FP32 build) on `steps` steps
-    of one shot of the workload."""
-    import orc
+def shortened(job, steps):
     j = dict(job)
     j["nt"] = steps
     j["rik"] = job["rik"][:steps]
+    return j
+
+
+def with_k2(job):
+    from cqpy import host
+    j = dict(job)
     if "k2d1" not in j:
-        from cqpy import host
-        j.update(host.k2d(j["dx"], j["dz"], j["nxx"], j["nzz"], q=False))
+        j.update(host.k2d(j["dx"], j["dz"], j["nxx"], j["nzz"], q=int(j["kind"]) in (1, 3)))
+    return j
+
+
+def cpu_rate(job, steps, threads):
+    """FP32 reference-faithful CPU path (the oracle's FP32 build: 9 / 14 full
+    2-D C2C FFTs per acoustic / viscoacoustic step, OpenMP) on `steps` steps
+    of one shot; returns (Mcell-steps/s, seconds)."""
+    import orc
+    j = with_k2(shortened(job, steps))
     orc.set_threads(threads, 32)
     t0 = time.perf_counter()
     orc.forward(j, bits=32)
     dt = time.perf_counter() - t0
     return j["nxx"] * j["nzz"] * steps / dt / 1e6, dt
+
+
+def host_threads():
+    """(threads used, nproc): OMP_NUM_THREADS where set (the GPU box sets
+    it to its CPU share), else the cores this process may run on."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    return (int(env) if env and env.isdigit() else nproc), nproc
+
+
+def cpu_bench(pair, steps_all=(48, 32), steps_one=(5, 4)):
+    """The CPU baseline on one acoustic and one viscoacoustic shot of the
+    workload, on all the process's cores and on one core.  The mix rate
+    weights the two kinds by equal cell-steps (the batch is half of each):
+    cells x steps summed over both, over the summed time."""
+    threads, nproc = host_threads()
+    res = {}
+    for label, n, steps in (("all", threads, steps_all), ("one", 1, steps_one)):
+        cells = secs = 0.0
+        for job, st in zip(pair, steps):
+            v, sec = cpu_rate(job, st, n)
+            res["%s_%s" % (("acoustic", "visco")[int(job["kind"])], label)] = round(v, 3)
+            cells += job["nxx"] * job["nzz"] * st
+            secs += sec
+        res["mix_" + label] = round(cells / secs / 1e6, 3)
+        res["seconds_" + label] = round(secs, 1)
+    return res, threads, nproc
+
+
+def plain_bench(ctx, pair, steps):
+    """The plain path (cq_plain_forward: one shot, one launch sequence per
+    step, the gather row copied to the host and the stream synchronised
+    after every step) on one acoustic and one viscoacoustic shot of the
+    workload; the mix rate weights them like the batch (equal cell-steps)."""
+    res, cells, secs = {}, 0.0, 0.0
+    for job in pair:
+        pj = shortened(job, steps)
+        t0 = time.perf_counter()
+        ctx.plain_forward(pj)
+        el = time.perf_counter() - t0
+        c = pj["nxx"] * pj["nzz"] * steps
+        res[("acoustic", "visco")[int(job["kind"])]] = round(c / el / 1e6, 1)
+        cells += c
+        secs += el
+    res["mix"] = round(cells / secs / 1e6, 1)
+    return res
+
+
+def accuracy_check(ctx, pair, steps):
+    """rel-L2 of the batched engine's gathers of one shot of each kind of the
+    workload against the FP64 oracle over the first `steps` steps."""
+    import orc
+    from cqpy import engine
+    jobs = [shortened(j, steps) for j in pair]
+    b = ctx.batch(jobs)
+    b.run()
+    got = [b.rec(i) for i in range(len(jobs))]
+    b.close()
+    threads, _ = host_threads()
+    orc.set_threads(threads, 64)
+    out = {}
+    for j, g in zip(jobs, got):
+        ref = orc.forward(with_k2(j))
+        out[("acoustic", "visco")[int(j["kind"])]] = float("%.3g" % engine.rel_l2(g, ref))
+    out["steps"] = steps
+    out["window"] = "%dx%d" % (jobs[0]["nxx"], jobs[0]["nzz"])
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "batch128_mixed_passes.md")
@@ -168,8 +246,9 @@ def serve_bench(rate, window_s, seed, chunk):
     allv = list(lat.values())
     return {"workload": "serve: Poisson %.0f jobs/s for %.1f s (%d jobs; 50%% acoustic, 30%% "
                         "viscoacoustic, 20%% Q-RTM with 40 checkpoints), windows %d-%d x %d-%d "
-                        "cells + 40-pt CPML, nt 2000-8000, dx 5-12.5 m, Ricker 10-30 Hz"
-                        % (rate, window_s, len(reqs), 512, 1216, 256, 600),
+                        "cells + 40-pt CPML (padded to 7-smooth sizes, at most %dx%d), nt 2000-8000, "
+                        "dx 5-12.5 m, Ricker 10-30 Hz"
+                        % (rate, window_s, len(reqs), 512, 4096, 256, 1024, mx, mz),
             "jobs": len(reqs), "wall_s": round(wall, 3),
             "useful_mcell_steps_per_s": round(useful / (wall - first) / 1e6, 1),
             "executed_mcell_steps_per_s": round(executed / (wall - first) / 1e6, 1),
@@ -224,7 +303,8 @@ def main():
     ap.add_argument("--kinds", default="mixed", choices=["mixed", "acoustic", "visco"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=24)
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a streaming copy of this many GB (counter calibration)")
@@ -260,9 +340,10 @@ def main():
                           "dt=0.4 ms, windows cropped to +-600 cells (uneven, widened to "
                           "FFT-friendly sizes)" % args.shots,
               "shots": args.shots,
-              "kinds": {"mixed": "half acoustic, half viscoacoustic (rank-4 fractional "
-                                 "Laplacian)", "acoustic": "acoustic",
-                        "visco": "viscoacoustic"}[args.kinds],
+              "kinds": {"mixed": "half acoustic, half viscoacoustic (the reference's rank-4 "
+                                 "Taylor-expanded fractional-Laplacian operator, sum_total)",
+                        "acoustic": "acoustic",
+                        "visco": "viscoacoustic (Taylor-expanded operator, sum_total)"}[args.kinds],
               "cells_per_step": int(sum(j["nxx"] * j["nzz"] for j in jobs))}
     else:
         jobs = [synth.config1_job(nt=max(nt_total, 3000), want_k2=False)]
@@ -300,17 +381,21 @@ def main():
     step_bytes = sum(algo.values())
     print("passes:", json.dumps(passes), file=sys.stderr, flush=True)
 
-    # plain path: one shot of the workload, per-step gather copy + sync
+    # one interior shot of each kind of the workload: plain path, accuracy, CPU
+    mid = len(jobs) // 2
+    pair = []
+    for k in (0, 1):
+        cand = [j for j in jobs[mid:] + jobs[:mid] if int(j["kind"]) == k]
+        if cand:
+            pair.append(cand[0])
     plain = None
     if not args.no_plain:
-        mid = jobs[len(jobs) // 2]
-        pj = dict(mid)
-        pj["nt"] = args.plain_steps
-        pj["rik"] = mid["rik"][:args.plain_steps]
-        t0 = time.perf_counter()
-        ctx.plain_forward(pj)
-        pel = time.perf_counter() - t0
-        plain = pj["nxx"] * pj["nzz"] * args.plain_steps / pel / 1e6
+        plain = plain_bench(ctx, pair, args.plain_steps)
+        print("plain:", json.dumps(plain), file=sys.stderr, flush=True)
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = accuracy_check(ctx, pair, args.accuracy_steps)
+        print("accuracy:", json.dumps(accuracy), file=sys.stderr, flush=True)
     ctx.close()
 
     serve = None
@@ -327,14 +412,14 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        mid = dict(jobs[len(jobs) // 2])
-        mid["kind"] = 0
-        v, sec = cpu_baseline(mid, args.cpu_steps, threads)
-        cpu = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-               "sample": "FP32 reference-faithful CPU path (acoustic: 9 full 2-D C2C FFTs per "
-                         "step, OpenMP) on one %dx%d shot of the workload, %d steps, %.1f s"
-                         % (mid["nxx"], mid["nzz"], args.cpu_steps, sec)}
+        res, threads, nproc = cpu_bench(pair)
+        cpu = {"value": res["mix_all"], "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+               "nproc": nproc, "one_core": res["mix_one"], "per_kind": res,
+               "sample": "FP32 reference-faithful CPU path (the oracle's FP32 build: 9 / 14 full 2-D "
+                         "C2C FFTs per acoustic / viscoacoustic step, OpenMP) on one %dx%d shot of each "
+                         "kind of the workload: 48 + 32 steps on %d threads (%.1f s), 5 + 4 steps on one "
+                         "core (%.1f s); value weights the kinds by equal cell-steps like the batch"
+                         % (pair[0]["nxx"], pair[0]["nzz"], threads, res["seconds_all"], res["seconds_one"])}
 
     out = {
         "metric": "Mcell-steps/s batched vs plain at rel-L2<=1e-4 vs FP64; p50/p99 shot latency",
@@ -350,8 +435,10 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded layered model with salt body; no field data)",
         "config": wl,
-        "plain_mcell_steps_per_s": None if plain is None else round(plain, 1),
-        "batched_over_plain": None if plain is None else round(value / plain, 2),
+        "plain_mcell_steps_per_s": None if plain is None else plain["mix"],
+        "plain": plain,
+        "batched_over_plain": None if plain is None else round(value / plain["mix"], 2),
+        "accuracy_rel_l2_vs_fp64": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(dom) if default_workload else None,

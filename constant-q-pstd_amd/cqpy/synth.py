@@ -199,7 +199,7 @@ def make_window_job(kind, vpt, rhot, nxt, nzt, dx, dz, dt, nt, fd, left_col, nx,
     return job
 
 
-def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, widen=True):
+def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, widen=True, depth_pad=0):
     """Batched multi-shot workload: 128 shots on a 600 x 2400 model, 40-point
     CPML, dx = dz = 5 m, dt = 0.4 ms, layered vp 1500-4500 m/s with a salt-like
     ellipse at 4500 m/s, Q 30-300, 20 Hz Ricker, shots every 18 cells at the
@@ -207,10 +207,17 @@ def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, wi
     Windows are cropped to +-600 cells around the source (uneven near the
     model edges; interior shots 1281 x 680 padded, 1281 = 3 7 61) and, with
     widen, widened to the right to the next 7-smooth padded width (1296).
+    depth_pad extends the model below its last row (replicated, as the
+    window rule replicates the model edge) so that the padded depth
+    600 + depth_pad + 80 has no large prime factor.
     kinds[i] = 0 (acoustic) or 1 (viscoacoustic)."""
     nxt, nzt, dx, pml = 2400, 600, 5.0, 40
     vp, rho, q = layered_model(nxt, nzt, dx, seed=301, nlayers=6, vmin=1500.0, vmax=4500.0,
                                qrange=(30.0, 300.0), salt=(1200, 330, 260, 90, 4500.0))
+    if depth_pad:
+        ext = lambda a: np.concatenate([a, np.repeat(a[:, -1:], depth_pad, axis=1)], axis=1)
+        vp, rho, q = ext(vp), ext(rho), ext(q)
+        nzt += depth_pad
     first = (nxt - 1 - 18 * (nshots - 1)) // 2
     jobs = []
     for i in range(nshots) if shots is None else shots:
@@ -230,7 +237,7 @@ def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, wi
 SERVE_CLASSES = ((0, 0.5), (1, 0.3), (3, 0.2))   # acoustic, viscoacoustic, Q-RTM
 
 
-def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 1216), nz_range=(256, 600),
+def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 4096), nz_range=(256, 1024),
                nt_range=(2000, 8000), dx_range=(5.0, 12.5), fd_range=(10.0, 30.0), pml=40, snaps=40):
     """Serving workload: Poisson arrivals (mean `rate` jobs/s) over `window_s`
     seconds of mixed requests, 50% acoustic forward, 30% viscoacoustic forward,
