@@ -38,6 +38,20 @@ class ServerDesc(C.Structure):
                 ("max_jobs", C.c_int)]
 
 
+class Tiles(C.Structure):
+    """cq_tiles: workgroup tile shapes of the passes and the FFT plan variant."""
+    _fields_ = [("tz", C.c_int), ("cw", C.c_int), ("cwq", C.c_int), ("tk", C.c_int), ("tk2", C.c_int),
+                ("plan", C.c_int)]
+
+    @classmethod
+    def of(cls, d):
+        """From a dict (missing fields: the engine's choice)."""
+        return cls(*(int(d.get(k, 0)) for k in ("tz", "cw", "cwq", "tk", "tk2")), int(d.get("plan", -1)))
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in ("tz", "cw", "cwq", "tk", "tk2", "plan")}
+
+
 QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 NPASS = 10   # CQ_NPASS
 
@@ -48,7 +62,7 @@ SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_creat
            "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image",
            "cq_server_open", "cq_server_submit", "cq_server_advance", "cq_server_poll",
            "cq_server_job_steps", "cq_server_result", "cq_server_release", "cq_server_close",
-           "cq_forward_static"]
+           "cq_forward_static", "cq_batch_create_tiled", "cq_batch_get_tiles", "cq_autotune"]
 
 _lib = None
 
@@ -154,8 +168,21 @@ class Context:
         _chk(lib().cq_forward_static(self.h, C.byref(d), rec.ctypes.data_as(_F)))
         return rec.reshape(int(job["nt"]), d.nr)
 
-    def batch(self, jobs):
-        return Batch(self, jobs)
+    def batch(self, jobs, tiles=None):
+        """A batch of the jobs; tiles: dict of cq_tiles fields (None: the
+        engine's choice)."""
+        return Batch(self, jobs, tiles)
+
+    def autotune(self, jobs, cache=None):
+        """cq_autotune: (tiles dict, ms per step, from_cache)."""
+        keep = []
+        arr = (JobDesc * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = make_desc(j, keep)
+        t, ms, hit = Tiles(), C.c_double(), C.c_int()
+        _chk(lib().cq_autotune(self.h, arr, C.c_int(len(jobs)), None if cache is None else str(cache).encode(),
+                               C.byref(t), C.byref(ms), C.byref(hit)))
+        return t.as_dict(), ms.value, bool(hit.value)
 
     def server(self, max_nxx, max_nzz, hbm_budget=0, max_jobs=0):
         return Server(self, max_nxx, max_nzz, hbm_budget, max_jobs)
@@ -168,7 +195,7 @@ class Context:
 
 
 class Batch:
-    def __init__(self, ctx, jobs):
+    def __init__(self, ctx, jobs, tiles=None):
         self.ctx = ctx
         self.jobs = jobs
         keep = []
@@ -176,8 +203,17 @@ class Batch:
         for i, j in enumerate(jobs):
             arr[i] = make_desc(j, keep)
         self.h = C.c_void_p()
-        _chk(lib().cq_batch_create(ctx.h, arr, C.c_int(len(jobs)), C.byref(self.h)))
+        if tiles is None:
+            _chk(lib().cq_batch_create(ctx.h, arr, C.c_int(len(jobs)), C.byref(self.h)))
+        else:
+            t = Tiles.of(tiles)
+            _chk(lib().cq_batch_create_tiled(ctx.h, arr, C.c_int(len(jobs)), C.byref(t), C.byref(self.h)))
         ctx.children.add(self)
+
+    def tiles(self):
+        t = Tiles()
+        _chk(lib().cq_batch_get_tiles(self.h, C.byref(t)))
+        return t.as_dict()
 
     def step(self, n):
         _chk(lib().cq_batch_step(self.h, C.c_int(n)))

@@ -34,7 +34,9 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <set>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -126,11 +128,13 @@ struct cq_ctx_s {
 };
 
 // Factorisation of n into supported radices for a workgroup that transforms
-// `need` points at once: the fewest stages, taking powers of two as 16/8/4/2
-// and powers of three as 9/3, then 5, 7, 11, 13, 17, but only radices whose
-// per-stage capacity (cq_stage_capacity) holds `need` points.  False if n
-// has another prime factor or a prime factor's radix is too small.
-static bool make_direct_plan(int n, int need, CqPlan &p)
+// `need` points at once.  Variant 0: the fewest stages, taking powers of two
+// as 16/8/4/2 and powers of three as 9/3, then 5, 7, 11, 13, 17; variant 1:
+// radices of at most 8 (powers of two as 8/4/2, powers of three as 3), more
+// stages of fewer registers each.  Only radices whose per-stage capacity
+// (cq_stage_capacity) holds `need` points are taken.  False if n has another
+// prime factor or a prime factor's radix is too small.
+static bool make_direct_plan(int n, int need, CqPlan &p, int variant = 0)
 {
     memset(&p, 0, sizeof p);
     p.n = n;
@@ -141,7 +145,9 @@ static bool make_direct_plan(int n, int need, CqPlan &p)
         m /= r;
         return true;
     };
-    for (int r : {16, 8, 4, 2, 9, 3, 5, 7, 11, 13, 17}) {
+    static const int kOrder[2][11] = {{16, 8, 4, 2, 9, 3, 5, 7, 11, 13, 17}, {8, 4, 2, 3, 5, 7, 11, 13, 17, 0, 0}};
+    for (int r : kOrder[variant == 1 ? 1 : 0]) {
+        if (r == 0) break;
         if (cq_stage_capacity(r) < need) continue;
         while (m % r == 0)
             if (!push(r)) return false;
@@ -174,20 +180,20 @@ static int fft_len(int n) { return smooth(n, {2, 3, 5, 7, 11, 13, 17}) ? n : blu
 
 // Plan of an n-point transform for `need` points (need / n lines) per
 // workgroup: direct, or Bluestein over fft_len(n) points per line.
-static bool make_plan(int n, int need, CqPlan &p)
+static bool make_plan(int n, int need, CqPlan &p, int variant = 0)
 {
     const int M = fft_len(n);
-    if (M == n) return make_direct_plan(n, need, p);
+    if (M == n) return make_direct_plan(n, need, p, variant);
     const int lines = (need + n - 1) / n;
-    if (!make_direct_plan(M, lines * M, p)) return false;
+    if (!make_direct_plan(M, lines * M, p, variant)) return false;
     p.bn = n;
     return true;
 }
 
-static bool has_plan(int n, int need)
+static bool has_plan(int n, int need, int variant = 0)
 {
     CqPlan p;
-    return make_plan(n, need, p);
+    return make_plan(n, need, p, variant);
 }
 
 // In-place FFT of a 7-smooth length in double precision (host), by
@@ -252,9 +258,9 @@ static int upload_table(cq_ctx ctx, cq_batch_s *pb, const std::vector<float2> &h
 // Plan and twiddle table of an n-point transform for `need` points per
 // workgroup (and a Bluestein plan's chirp tables).  The tables are cached in
 // the context, or, for a pooled batch, placed in its pool.
-static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, CqPlan &p, const float2 *&tw)
+static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, int variant, CqPlan &p, const float2 *&tw)
 {
-    if (!make_plan(n, need, p))
+    if (!make_plan(n, need, p, variant))
         return set_err(CQ_EUNSUPPORTED,
                        "FFT length %d (transform length %d): no factorisation into radices 2..17 for %d points "
                        "per workgroup", n, fft_len(n), need / n * fft_len(n));
@@ -491,7 +497,8 @@ struct cq_batch_s {
     int nitems[2][P_COUNT] = {};
     size_t lds[P_COUNT] = {};
     int tz = 16, cw = 16, cwq = 16, tk = 8;   // row, acoustic column, Q column, kz-row tiles
-    int tk2 = 8;                              // kz-row tile of k_q_r2 (tk, 2 tk or 3 tk)
+    int tk2 = 8;                              // kz-row tile of k_q_r2 (tk, 2 tk or 3 tk by default)
+    int plan = 0;                             // FFT factorisation variant (make_direct_plan)
     int maxnxx = 0, maxnzz = 0;
     int t = 0, tmax = 0;                   // next global step; end of the last wavefield
     bool timing = false;
@@ -631,7 +638,7 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
     }
 }
 
-static bool r1_fused_for(int nxx, int tk);
+static bool r1_fused_for(int nxx, int tk, int variant);
 static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out);
 
 // Allocates and uploads one shot for the batch's tile shapes.  With
@@ -644,7 +651,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = ((nzz + tz - 1) / tz * tz + 31) & ~31;   // whole row tiles, 128-B rows
     const int nzh = nzz / 2 + 1;
-    const int ldk = (nzh + tk2 - 1) / tk2 * tk2;   // whole tiles of both kz-row passes (tk | tk2)
+    const int kt = std::lcm(tk, tk2);
+    const int ldk = (nzh + kt - 1) / kt * kt;   // whole tiles of both kz-row passes
     const int nx = nxx - 2 * pml, nz = nzz - 2 * pml;
     const size_t N = (size_t)nxx * ldz;
     const int ntile_x = (nxx + cw - 1) / cw, ntile_z = (nzz + tz - 1) / tz;
@@ -664,7 +672,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.qsign = (j.kind == CQ_KIND_QRTM) ? -1.0f : 1.0f;
     base.stab = (j.kind == CQ_KIND_QRTM) ? 1 : 0;
     base.gather = rtm ? 0 : 1;
-    base.r1_fused = q && r1_fused_for(nxx, tk) ? 1 : 0;
+    base.r1_fused = q && r1_fused_for(nxx, tk, b->plan) ? 1 : 0;
     int rc;
     // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 2 tk
     // (fused k_q_r1) or tk lines and tk2 lines (k_q_r2) of nxx; column tiles
@@ -673,12 +681,12 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
               need_z = (cw / 2) * nzz;
     if (bytes_only) {
         // footprint only: the plans must exist, nothing is uploaded
-        if (!make_plan(nxx, need_x, base.px) || !make_plan(nzz, need_z, base.pz))
+        if (!make_plan(nxx, need_x, base.px, b->plan) || !make_plan(nzz, need_z, base.pz, b->plan))
             return set_err(CQ_EUNSUPPORTED, "window %dx%d: no FFT plan for the batch's tiles", nxx, nzz);
     } else {
         cq_batch_s *pb = b->pool ? b : nullptr;
-        if ((rc = get_plan(ctx, pb, nxx, need_x, base.px, base.twx)) != CQ_OK) return rc;
-        if ((rc = get_plan(ctx, pb, nzz, need_z, base.pz, base.twz)) != CQ_OK) return rc;
+        if ((rc = get_plan(ctx, pb, nxx, need_x, b->plan, base.px, base.twx)) != CQ_OK) return rc;
+        if ((rc = get_plan(ctx, pb, nzz, need_z, b->plan, base.pz, base.twz)) != CQ_OK) return rc;
     }
     // lines hold the transform length (M points for a Bluestein plan)
     base.twlx = base.px.n;
@@ -984,15 +992,18 @@ static const int kTkCand[] = {8, 4, 2, 1};
 // k_q_r1 holds 3 tk lines and transforms 2 tk at a time (the div and p
 // spectra, then two symbols at a time) when the LDS budget allows it, else
 // inverts one symbol at a time (2 tk lines: spectrum + work, tk at a time).
-static bool r1_fused_for(int nxx, int tk)
+static bool r1_fused_for(int nxx, int tk, int variant)
 {
-    return has_plan(nxx, 2 * tk * nxx) && lds_lines(nxx, 3 * tk) <= kLdsTile;
+    return has_plan(nxx, 2 * tk * nxx, variant) && lds_lines(nxx, 3 * tk) <= kLdsTile;
 }
-static size_t r1_lds(int nxx, int tk) { return lds_lines(nxx, r1_fused_for(nxx, tk) ? 3 * tk : 2 * tk); }
-static bool r1_ok(int nxx, int tk, size_t lim)
+static size_t r1_lds(int nxx, int tk, int variant)
 {
-    return (has_plan(nxx, 2 * tk * nxx) && lds_lines(nxx, 3 * tk) <= lim) ||
-           (has_plan(nxx, tk * nxx) && lds_lines(nxx, 2 * tk) <= lim);
+    return lds_lines(nxx, r1_fused_for(nxx, tk, variant) ? 3 * tk : 2 * tk);
+}
+static bool r1_ok(int nxx, int tk, size_t lim, int variant = 0)
+{
+    return (has_plan(nxx, 2 * tk * nxx, variant) && lds_lines(nxx, 3 * tk) <= lim) ||
+           (has_plan(nxx, tk * nxx, variant) && lds_lines(nxx, 2 * tk) <= lim);
 }
 
 // LDS of each pass for one window of the given kind with the batch's tiles.
@@ -1004,7 +1015,7 @@ static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out)
         out[P_ACOL] = lds_lines(nzz, b->cw / 2);
     } else {
         out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = lds_lines(nzz, b->cwq / 2);
-        out[P_QR1] = r1_lds(nxx, b->tk);
+        out[P_QR1] = r1_lds(nxx, b->tk, b->plan);
         out[P_LR1] = lds_lines(nxx, 2 * b->tk);
         out[P_QR2] = lds_lines(nxx, b->tk2);
         out[P_QR3] = lds_lines(nxx, b->tz / 2);
@@ -1019,7 +1030,7 @@ static void set_lds(cq_batch b, int ax, int az, int qx, int qz, const std::vecto
     b->lds[P_ACOL] = lds_lines(az, b->cw / 2);
     b->lds[P_QC1] = lds_lines(qz, b->cwq / 2);
     b->lds[P_QR1] = 0;
-    for (int n : qxs) b->lds[P_QR1] = std::max(b->lds[P_QR1], r1_lds(n, b->tk));
+    for (int n : qxs) b->lds[P_QR1] = std::max(b->lds[P_QR1], r1_lds(n, b->tk, b->plan));
     b->lds[P_QC2] = lds_lines(qz, b->cwq / 2);
     b->lds[P_QR2] = lds_lines(qx, b->tk2);
     b->lds[P_QC3] = lds_lines(qz, b->cwq / 2);
@@ -1180,7 +1191,71 @@ static int rebuild_tables(cq_batch b)
     return CQ_OK;
 }
 
+// Candidate k_q_r2 tiles (instantiated widths).
+static const int kTk2Cand[] = {16, 12, 8, 6, 4, 3, 2, 1};
+
+template <size_t N>
+static bool in_set(int v, const int (&set)[N])
+{
+    return std::find(set, set + N, v) != set + N;
+}
+
+// The caller's tile fields over the engine's choice (choose_tiles has run):
+// each value must be an instantiated width and every window's transforms
+// must fit it (a plan of the batch's variant, at most kLdsMax of LDS).
+static int apply_tiles(cq_batch b, const std::vector<TileReq> &req, const cq_tiles *t)
+{
+    if (!t) return CQ_OK;
+    if ((t->tz && !in_set(t->tz, kTzCand)) || (t->cw && !in_set(t->cw, kCwCand)) ||
+        (t->cwq && !in_set(t->cwq, kCwCand)) || (t->tk && !in_set(t->tk, kTkCand)) ||
+        (t->tk2 && !in_set(t->tk2, kTk2Cand)) || t->plan < -1 || t->plan > 1)
+        return set_err(CQ_EINVAL, "tiles tz=%d cw=%d cwq=%d tk=%d tk2=%d plan=%d: a value outside its set", t->tz,
+                       t->cw, t->cwq, t->tk, t->tk2, t->plan);
+    if (t->tz) b->tz = t->tz;
+    if (t->cw) b->cw = t->cw;
+    if (t->cwq) b->cwq = t->cwq;
+    if (t->tk) b->tk = t->tk;
+    if (t->tk2) b->tk2 = t->tk2;
+    if (t->plan >= 0) b->plan = t->plan;
+    const int v = b->plan;
+    auto fits = [&](int n, int lines) { return has_plan(n, lines * n, v) && lds_lines(n, lines) <= kLdsMax; };
+    int ax = 0, az = 0, qx = 0, qz = 0;
+    std::vector<int> qxs;
+    for (size_t i = 0; i < req.size(); i++) {
+        const TileReq &j = req[i];
+        bool ok = fits(j.nxx, b->tz / 2) && fits(j.nzz, (j.q ? b->cwq : b->cw) / 2);
+        if (j.q) ok = ok && r1_ok(j.nxx, b->tk, kLdsMax, v) && fits(j.nxx, b->tk2);
+        if (!ok)
+            return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d does not fit tiles tz=%d cw=%d cwq=%d tk=%d tk2=%d "
+                           "plan=%d", (int)i, j.nxx, j.nzz, b->tz, b->cw, b->cwq, b->tk, b->tk2, v);
+        int &mx = j.q ? qx : ax, &mz = j.q ? qz : az;
+        mx = std::max(mx, j.nxx);
+        mz = std::max(mz, j.nzz);
+        if (j.q) qxs.push_back(j.nxx);
+    }
+    set_lds(b, ax, az, qx, qz, qxs);
+    return CQ_OK;
+}
+
 extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, cq_batch *out)
+{
+    return cq_batch_create_tiled(ctx, jobs, njobs, nullptr, out);
+}
+
+extern "C" int cq_batch_get_tiles(cq_batch b, cq_tiles *out)
+{
+    if (!b || !out) return set_err(CQ_EINVAL, "cq_batch_get_tiles: bad arguments");
+    out->tz = b->tz;
+    out->cw = b->cw;
+    out->cwq = b->cwq;
+    out->tk = b->tk;
+    out->tk2 = b->tk2;
+    out->plan = b->plan;
+    return CQ_OK;
+}
+
+extern "C" int cq_batch_create_tiled(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const cq_tiles *tiles,
+                                     cq_batch *out)
 {
     if (!ctx || !jobs || njobs < 1 || !out) return set_err(CQ_EINVAL, "cq_batch_create: bad arguments");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1193,6 +1268,7 @@ extern "C" int cq_batch_create(cq_ctx ctx, const cq_job_desc *jobs, int njobs, c
     std::vector<TileReq> req;
     for (int i = 0; i < njobs; i++) req.push_back({jobs[i].nxx, jobs[i].nzz, is_q(jobs[i].kind)});
     int rc = choose_tiles(b.get(), req);
+    if (rc == CQ_OK) rc = apply_tiles(b.get(), req, tiles);
     if (rc) return rc;
     b->jobs.resize(njobs);
     for (int i = 0; i < njobs; i++) {
@@ -2054,4 +2130,245 @@ extern "C" int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec)
     if (rc == CQ_OK) rc = read_rec(ctx, b.jobs[0], rec);
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner (cq_autotune)
+// ---------------------------------------------------------------------------
+// Cache key of a job set: library version, device architecture and CU count,
+// then every window class (a: acoustic or RTM, q: viscoacoustic or Q-RTM,
+// l<r>: low-rank of rank r) with its padded size and count, sorted.
+static std::string tune_key(cq_ctx ctx, const cq_job_desc *jobs, int njobs)
+{
+    std::map<std::string, int> win;
+    char buf[64];
+    for (int i = 0; i < njobs; i++) {
+        const cq_job_desc &j = jobs[i];
+        if (!is_q(j.kind)) snprintf(buf, sizeof buf, "a%dx%d", j.nxx, j.nzz);
+        else if (j.lr_rank > 0) snprintf(buf, sizeof buf, "l%d:%dx%d", j.lr_rank, j.nxx, j.nzz);
+        else snprintf(buf, sizeof buf, "q%dx%d", j.nxx, j.nzz);
+        win[buf]++;
+    }
+    std::string arch = "unknown";
+    int cus = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+    }
+    std::string key = std::string(CQ_VERSION) + "|" + arch + "|" + std::to_string(cus) + "|";
+    for (auto &kv : win) key += kv.first + "*" + std::to_string(kv.second) + ",";
+    for (char &c : key)
+        if (c == ' ' || c == '\t' || c == '\n' || c == '\r') c = '_';
+    return key;
+}
+
+// Cache file: one line per tuned job set, "key<TAB>tz cw cwq tk tk2 plan ms";
+// '#' lines are comments; the last line of a key wins.
+static bool tune_lookup(const char *path, const std::string &key, cq_tiles &t, double &ms)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    bool found = false;
+    std::string line;
+    int c;
+    for (;;) {
+        line.clear();
+        while ((c = fgetc(f)) != EOF && c != '\n') line.push_back((char)c);
+        if (!line.empty() && line[0] != '#') {
+            const size_t tab = line.find('\t');
+            cq_tiles x;
+            double m;
+            if (tab != std::string::npos && line.compare(0, tab, key) == 0 && tab == key.size() &&
+                sscanf(line.c_str() + tab + 1, "%d %d %d %d %d %d %lf", &x.tz, &x.cw, &x.cwq, &x.tk, &x.tk2, &x.plan,
+                       &m) == 7) {
+                t = x;
+                ms = m;
+                found = true;
+            }
+        }
+        if (c == EOF) break;
+    }
+    fclose(f);
+    return found;
+}
+
+static int tune_store(const char *path, const std::string &key, const cq_tiles &t, double ms)
+{
+    FILE *f = fopen(path, "a");
+    if (!f) return set_err(CQ_EINVAL, "cq_autotune: cannot write the cache %s", path);
+    if (ftell(f) == 0) fprintf(f, "# cq_autotune cache: key<TAB>tz cw cwq tk tk2 plan ms_per_step\n");
+    fprintf(f, "%s\t%d %d %d %d %d %d %.6f\n", key.c_str(), t.tz, t.cw, t.cwq, t.tk, t.tk2, t.plan, ms);
+    fclose(f);
+    return CQ_OK;
+}
+
+static const int kTuneJobs = 24;   // trial batch: at most this many jobs
+static const int kTuneWarm = 2, kTuneMeas = 6;
+
+// The trial batch's jobs: every job up to kTuneJobs, else one of each window
+// class and an even spread of the rest; RTM kinds run as their modeling
+// kinds (the same passes), every job for kTuneWarm + kTuneMeas steps.
+static std::vector<cq_job_desc> tune_jobs(const cq_job_desc *jobs, int njobs)
+{
+    std::vector<int> pick;
+    if (njobs <= kTuneJobs) {
+        for (int i = 0; i < njobs; i++) pick.push_back(i);
+    } else {
+        std::set<std::tuple<bool, int, int, int>> seen;
+        std::vector<bool> used(njobs, false);
+        for (int i = 0; i < njobs; i++) {
+            const auto k = std::make_tuple(is_q(jobs[i].kind), jobs[i].lr_rank, jobs[i].nxx, jobs[i].nzz);
+            if (seen.insert(k).second) {
+                pick.push_back(i);
+                used[i] = true;
+            }
+        }
+        const int room = kTuneJobs - (int)pick.size();
+        for (int r = 0; r < room; r++) {
+            const int i = (int)((long long)r * njobs / std::max(room, 1));
+            if (!used[i]) {
+                pick.push_back(i);
+                used[i] = true;
+            }
+        }
+    }
+    std::vector<cq_job_desc> out;
+    for (int i : pick) {
+        cq_job_desc d = jobs[i];
+        if (d.kind == CQ_KIND_RTM) d.kind = CQ_KIND_ACOUSTIC;
+        if (d.kind == CQ_KIND_QRTM) d.kind = CQ_KIND_VISCO;
+        d.nt = std::min(d.nt, kTuneWarm + kTuneMeas);
+        out.push_back(d);
+    }
+    return out;
+}
+
+// One trial: a batch of the trial jobs with tiles t, kTuneWarm steps, then
+// kTuneMeas steps timed per pass (pm != null, per-pass events on one stream)
+// or as whole steps (step_ms, the launch groups overlapping).
+static int tune_trial(cq_ctx ctx, const std::vector<cq_job_desc> &tj, const cq_tiles &t, double *pm, double *step_ms)
+{
+    cq_batch b = nullptr;
+    int rc = cq_batch_create_tiled(ctx, tj.data(), (int)tj.size(), &t, &b);
+    if (rc) return rc;
+    rc = cq_batch_step(b, kTuneWarm);
+    if (rc == CQ_OK) rc = cq_batch_sync(b);
+    if (rc == CQ_OK && pm) {
+        int calls[P_COUNT];
+        rc = cq_batch_enable_timing(b, 1);
+        if (rc == CQ_OK) rc = cq_batch_step(b, kTuneMeas);
+        if (rc == CQ_OK) rc = cq_batch_kernel_times(b, pm, calls);
+    } else if (rc == CQ_OK) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = set_err(CQ_EDEVICE, "cq_autotune: hipEventCreate failed");
+        } else {
+            float ms = 0.f;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = set_err(CQ_EDEVICE, "cq_autotune: event record");
+            if (rc == CQ_OK) rc = cq_batch_step(b, kTuneMeas);
+            if (rc == CQ_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = set_err(CQ_EDEVICE, "cq_autotune: event timing failed");
+            *step_ms = ms / kTuneMeas;
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    cq_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int cq_autotune(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const char *cache, cq_tiles *out,
+                           double *tuned_ms, int *from_cache)
+{
+    if (!ctx || !jobs || njobs < 1 || !out) return set_err(CQ_EINVAL, "cq_autotune: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int i = 0; i < njobs; i++) {
+        const int rc = validate(jobs[i], i);
+        if (rc) return rc;
+    }
+    const std::string key = tune_key(ctx, jobs, njobs);
+    double ms = 0.0;
+    if (from_cache) *from_cache = 0;
+    if (cache && tune_lookup(cache, key, *out, ms)) {
+        if (tuned_ms) *tuned_ms = ms;
+        if (from_cache) *from_cache = 1;
+        return CQ_OK;
+    }
+    const std::vector<cq_job_desc> tj = tune_jobs(jobs, njobs);
+    bool has_a = false, has_q = false;
+    for (const cq_job_desc &d : tj) (is_q(d.kind) ? has_q : has_a) = true;
+    // the engine's own choice, per-pass times as the reference point
+    cq_tiles base;
+    {
+        cq_batch b = nullptr;
+        int rc = cq_batch_create_tiled(ctx, tj.data(), (int)tj.size(), nullptr, &b);
+        if (rc) return rc;
+        cq_batch_get_tiles(b, &base);
+        cq_batch_destroy(b);
+    }
+    double bpm[P_COUNT];
+    int rc = tune_trial(ctx, tj, base, bpm, nullptr);
+    if (rc) return rc;
+    // one field at a time from the engine's choice; a candidate replaces the
+    // current value when the passes it shapes run at least 1% faster
+    struct Field {
+        int cq_tiles::*m;
+        const int *cand;
+        int ncand;
+        std::vector<int> passes;
+        bool used;
+    };
+    const Field fields[] = {
+        {&cq_tiles::tz, kTzCand, (int)(sizeof kTzCand / sizeof(int)), {P_AROW, P_QR3}, true},
+        {&cq_tiles::cw, kCwCand, (int)(sizeof kCwCand / sizeof(int)), {P_ACOL}, has_a},
+        {&cq_tiles::cwq, kCwCand, (int)(sizeof kCwCand / sizeof(int)), {P_QC1, P_QC2, P_LR2, P_QC3}, has_q},
+        {&cq_tiles::tk, kTkCand, (int)(sizeof kTkCand / sizeof(int)), {P_QR1, P_LR1}, has_q},
+        {&cq_tiles::tk2, kTk2Cand, (int)(sizeof kTk2Cand / sizeof(int)), {P_QR2}, has_q},
+    };
+    cq_tiles best = base;
+    for (const Field &f : fields) {
+        if (!f.used) continue;
+        double best_t = 0.0;
+        for (int p : f.passes) best_t += bpm[p];
+        for (int c = 0; c < f.ncand; c++) {
+            if (f.cand[c] == base.*f.m) continue;
+            cq_tiles t = base;
+            t.*f.m = f.cand[c];
+            double pm[P_COUNT];
+            if (tune_trial(ctx, tj, t, pm, nullptr) != CQ_OK) continue;   // does not fit: skipped
+            double tt = 0.0;
+            for (int p : f.passes) tt += pm[p];
+            if (tt < 0.99 * best_t) {
+                best_t = tt;
+                best.*f.m = f.cand[c];
+            }
+        }
+    }
+    // whole steps: the engine's choice, the per-field winners, and the
+    // winners with the other FFT factorisation; the fastest that runs wins
+    cq_tiles cands[3] = {base, best, best};
+    cands[2].plan = 1 - best.plan;
+    cq_tiles win = base;
+    double win_ms = 0.0;
+    bool have = false;
+    for (const cq_tiles &t : cands) {
+        double sm = 0.0;
+        if (tune_trial(ctx, tj, t, nullptr, &sm) != CQ_OK) continue;
+        if (!have || sm < win_ms) {
+            win = t;
+            win_ms = sm;
+            have = true;
+        }
+    }
+    if (!have) return set_err(CQ_EDEVICE, "cq_autotune: no trial ran");
+    *out = win;
+    if (tuned_ms) *tuned_ms = win_ms;
+    if (cache) {
+        rc = tune_store(cache, key, win, win_ms);
+        if (rc) return rc;
+    }
+    g_err.clear();
+    return CQ_OK;
 }

@@ -193,6 +193,41 @@ int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec);
  * contexts, race and must be serialised by the caller. */
 int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec);
 
+/* Tile shapes and FFT plans of a batch (no counterpart in the reference, whose
+ * kernels take fixed CUDA block shapes and cuFFT plans, sgpstd-qmodeling/
+ * acps4.cu:871-872).  Every pass runs one workgroup per tile:
+ *   tz   depths per row tile of k_row_ac and k_q_r3: 4, 8, 12 or 16
+ *   cw   columns per tile of k_col_ac: 2, 4, 8, 12, 16, 24 or 32
+ *   cwq  columns per tile of the viscoacoustic column passes (k_q_c1, k_q_c2,
+ *        k_q_c3, k_lr_c2): the same values
+ *   tk   kz rows per tile of k_q_r1 and k_lr_r1: 1, 2, 4 or 8
+ *   tk2  kz rows per tile of k_q_r2: 1, 2, 3, 4, 6, 8, 12 or 16
+ *   plan FFT factorisation: 0 largest radices first (powers of two as 16,
+ *        powers of three as 9), 1 radices of at most 8 (no 16 or 9)
+ * A field of 0 (plan: -1) leaves that choice to the engine. */
+typedef struct {
+    int tz, cw, cwq, tk, tk2, plan;
+} cq_tiles;
+
+/* cq_batch_create with the given tiles (NULL: the engine's choice).  Fails
+ * with CQ_EUNSUPPORTED if a window's transforms do not fit the tiles. */
+int cq_batch_create_tiled(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const cq_tiles *tiles, cq_batch *out);
+/* The tiles a batch runs with. */
+int cq_batch_get_tiles(cq_batch b, cq_tiles *out);
+
+/* Autotuner: the tiles and FFT plan that run a batch of these jobs fastest on
+ * this device.  Looks the windows (kind class and padded size of every job)
+ * up in the text file `cache` (NULL: no cache); on a miss it times a few
+ * steps of trial batches of the same windows -- every candidate of each
+ * tile field and each plan, per-pass device times, then the whole step of
+ * the best combination against the engine's own choice -- and appends the
+ * winner to the cache.  The jobs themselves are not advanced (the trials use
+ * copies of their descriptions with at most 8 steps).  *tuned_ms receives
+ * the winner's ms per global step (optional); *from_cache 1 if the cache
+ * answered (optional). */
+int cq_autotune(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const char *cache, cq_tiles *out,
+                double *tuned_ms, int *from_cache);
+
 /* Per-pass device timing with HIP events on the engine's stream.  While
  * enabled, every launch of cq_batch_step is bracketed by events; passes are
  * numbered 0..CQ_NPASS-1 (cq_pass_name gives the kernel name): acoustic row,
