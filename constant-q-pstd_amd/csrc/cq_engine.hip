@@ -70,11 +70,27 @@ static int set_err(int code, const char *fmt, ...)
             return set_err(CQ_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-// Streaming copy used to calibrate HBM byte counters and bandwidth.
+// Streaming copy used to calibrate HBM byte counters and to measure the
+// bandwidth a streaming kernel attains: one block per 16 KB, each lane's
+// four 16-byte loads issued before its stores, nontemporal both ways (as the
+// passes stream their fields).
+typedef float cq_cv4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_copy_f4(const float4 *__restrict__ a, float4 *__restrict__ b, long long n4)
 {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    const cq_cv4 *src = reinterpret_cast<const cq_cv4 *>(a);
+    cq_cv4 *dst = reinterpret_cast<cq_cv4 *>(b);
+    cq_cv4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const long long i = base + u * 256;
+        if (i < n4) v[u] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const long long i = base + u * 256;
+        if (i < n4) __builtin_nontemporal_store(v[u], dst + i);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -180,14 +196,31 @@ static int fft_len(int n) { return smooth(n, {2, 3, 5, 7, 11, 13, 17}) ? n : blu
 
 // Plan of an n-point transform for `need` points (need / n lines) per
 // workgroup: direct, or Bluestein over fft_len(n) points per line.
+// Transforms longer than this keep a compressed last-stage twiddle table
+// (CqPlan::tc): for them the full table would take LDS from the lines.
+static const int kTcMin = 2048;
+
 static bool make_plan(int n, int need, CqPlan &p, int variant = 0)
 {
+    if (n < 1) return false;
     const int M = fft_len(n);
-    if (M == n) return make_direct_plan(n, need, p, variant);
-    const int lines = (need + n - 1) / n;
-    if (!make_direct_plan(M, lines * M, p, variant)) return false;
-    p.bn = n;
+    if (M == n) {
+        if (!make_direct_plan(n, need, p, variant)) return false;
+    } else {
+        const int lines = (need + n - 1) / n;
+        if (!make_direct_plan(M, lines * M, p, variant)) return false;
+        p.bn = n;
+    }
+    p.tc = p.n > kTcMin && p.ns > 1 ? 1 : 0;
     return true;
+}
+
+// Entries of a plan's twiddle table: n (n - 1 used), or 2 n / R - 1 with a
+// compressed last stage of radix R (its Ns = n / R first powers after the
+// n / R - 1 entries of the earlier stages).
+static int tw_len(const CqPlan &p)
+{
+    return p.tc ? 2 * (p.n / p.rad[p.ns - 1]) - 1 : p.n;
 }
 
 static bool has_plan(int n, int need, int variant = 0)
@@ -282,20 +315,22 @@ static int get_plan(cq_ctx ctx, cq_batch_s *pb, int n, int need, int variant, Cq
             p.bspec = dB;
         }
     }
-    const int N = p.n;   // points of the stages' transforms (M for Bluestein)
     std::vector<int> key(p.rad, p.rad + p.ns);
+    if (p.tc) key.push_back(-1);
     auto itp = ctx->twiddles.find(key);
     if (!pb && itp != ctx->twiddles.end()) {
         tw = itp->second;
         return CQ_OK;
     }
     // per-stage tables (cq_fft.h): stage s holds w_{Ns R}^(r k) at
-    // (r-1) Ns + k, r = 1..R-1, k < Ns; N - 1 entries in all (one spare)
-    std::vector<float2> h(N, make_float2(1.f, 0.f));
+    // (r-1) Ns + k, r = 1..R-1, k < Ns; n - 1 entries in all (one spare);
+    // a compressed last stage holds r = 1 only
+    std::vector<float2> h(tw_len(p), make_float2(1.f, 0.f));
     size_t o = 0;
     for (int s = 0, Ns = 1; s < p.ns; Ns *= p.rad[s], s++) {
         const int R = p.rad[s];
-        for (int r = 1; r < R; r++)
+        const int rmax = (p.tc && s == p.ns - 1) ? 2 : R;
+        for (int r = 1; r < rmax; r++)
             for (int k = 0; k < Ns; k++) {
                 const double a = -2.0 * M_PI * (double)r * (double)k / ((double)Ns * R);
                 h[o + (size_t)(r - 1) * Ns + k] = make_float2((float)cos(a), (float)sin(a));
@@ -639,7 +674,7 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
 }
 
 static bool r1_fused_for(int nxx, int tk, int variant);
-static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out);
+static void job_lds(cq_batch b, const JobD &d, bool q, size_t *out);
 
 // Allocates and uploads one shot for the batch's tile shapes.  With
 // bytes_only != null only its device footprint is computed.
@@ -689,15 +724,15 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         if ((rc = get_plan(ctx, pb, nzz, need_z, b->plan, base.pz, base.twz)) != CQ_OK) return rc;
     }
     // lines hold the transform length (M points for a Bluestein plan)
-    base.twlx = base.px.n;
-    base.twlz = base.pz.n;
+    base.twlx = tw_len(base.px);
+    base.twlz = tw_len(base.pz);
     base.lsx = cq_line_stride(base.px.n);
     base.lsz = cq_line_stride(base.pz.n);
     {
         // the passes' LDS for this window with the batch's tiles; a
         // Bluestein length may need more than the batch's widest window
         size_t need_lds[P_COUNT];
-        job_lds(b, nxx, nzz, q, need_lds);
+        job_lds(b, base, q, need_lds);
         for (int pp = 0; pp < P_COUNT; pp++)
             if (need_lds[pp] > kLdsMax)
                 return set_err(CQ_EUNSUPPORTED, "window %dx%d: pass %s needs %zu bytes of LDS (at most %zu)", nxx, nzz,
@@ -975,12 +1010,17 @@ static std::vector<int2> xcd_order(const std::vector<int2> &items)
 
 // LDS bytes of a workgroup holding the twiddle table of an n-point transform
 // and `lines` lines of it (cq_line_stride slots apart), for the transform
-// length fft_len(n) (M points for a Bluestein length).
+// length fft_len(n) (M points for a Bluestein length) and the plan of that
+// many lines (its table length); no plan: more than any workgroup has.
 static size_t lds_lines(int n, int lines)
 {
-    const int m = fft_len(n);
-    return (size_t)(m + lines * cq_line_stride(m)) * 8;
+    if (n < 1) return 0;   // no window of this kind
+    CqPlan p;
+    if (!make_plan(n, lines * n, p)) return (size_t)1 << 40;
+    return (size_t)(tw_len(p) + lines * cq_line_stride(p.n)) * 8;
 }
+// The same for a job's own plan (twl table entries, line stride ls).
+static size_t lds_job(int twl, int ls, int lines) { return (size_t)(twl + lines * ls) * 8; }
 // Candidate tile widths, largest first: z rows per row tile (two per packed
 // line), columns per column tile (two per packed line), kz rows per kz-row
 // tile.  Widths that are not powers of two fill a workgroup's 512 threads
@@ -1006,19 +1046,22 @@ static bool r1_ok(int nxx, int tk, size_t lim, int variant = 0)
            (has_plan(nxx, tk * nxx, variant) && lds_lines(nxx, 2 * tk) <= lim);
 }
 
-// LDS of each pass for one window of the given kind with the batch's tiles.
-static void job_lds(cq_batch b, int nxx, int nzz, bool q, size_t *out)
+// LDS of each pass for one job (its plans' table lengths and line strides)
+// with the batch's tiles.
+static void job_lds(cq_batch b, const JobD &d, bool q, size_t *out)
 {
     for (int p = 0; p < P_COUNT; p++) out[p] = 0;
+    auto xl = [&](int lines) { return lds_job(d.twlx, d.lsx, lines); };
+    auto zl = [&](int lines) { return lds_job(d.twlz, d.lsz, lines); };
     if (!q) {
-        out[P_AROW] = lds_lines(nxx, b->tz / 2);
-        out[P_ACOL] = lds_lines(nzz, b->cw / 2);
+        out[P_AROW] = xl(b->tz / 2);
+        out[P_ACOL] = zl(b->cw / 2);
     } else {
-        out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = lds_lines(nzz, b->cwq / 2);
-        out[P_QR1] = r1_lds(nxx, b->tk, b->plan);
-        out[P_LR1] = lds_lines(nxx, 2 * b->tk);
-        out[P_QR2] = lds_lines(nxx, b->tk2);
-        out[P_QR3] = lds_lines(nxx, b->tz / 2);
+        out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = zl(b->cwq / 2);
+        out[P_QR1] = xl(d.r1_fused ? 3 * b->tk : 2 * b->tk);
+        out[P_LR1] = xl(2 * b->tk);
+        out[P_QR2] = xl(b->tk2);
+        out[P_QR3] = xl(b->tz / 2);
     }
 }
 
@@ -2046,9 +2089,10 @@ extern "C" int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipMemsetAsync(a, 0, n4 * 16, ctx->stream));
-    hipLaunchKernelGGL(k_copy_f4, dim3(4096), dim3(256), 0, ctx->stream, a, b, n4);
+    const unsigned grid = (unsigned)((n4 + 1023) / 1024);
+    hipLaunchKernelGGL(k_copy_f4, dim3(grid), dim3(256), 0, ctx->stream, a, b, n4);
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_f4, dim3(4096), dim3(256), 0, ctx->stream, a, b, n4);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_f4, dim3(grid), dim3(256), 0, ctx->stream, a, b, n4);
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;

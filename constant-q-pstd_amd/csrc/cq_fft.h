@@ -44,11 +44,16 @@ __host__ __device__ constexpr int cq_nb(int R)
 // then n is the smooth length M >= 2 bn - 1 of the inner transforms, bn the
 // true length, chirp[j] = exp(-i pi j^2 / bn) (j < bn) and bspec the M-point
 // spectrum of the conjugate chirp, wrapped cyclically, divided by M.
+// tc: the last stage's twiddles are stored as their first powers only
+// (w^k, k < Ns) and raised to the powers r = 2 .. R-1 in registers, which
+// shrinks the table from n - 1 to 2 n / R - 1 entries (long lines, whose
+// tables would otherwise take LDS from the lines).
 struct CqPlan {
     int n;
     int ns;
     int rad[CQ_MAXST];
     int bn;   // 0: direct transform of n points
+    int tc;
     const float2 *chirp;   // device pointers (global memory)
     const float2 *bspec;
 };
@@ -261,7 +266,8 @@ extern __shared__ float2 cq_lds[];
 // MODE : 0 values in order on both sides; 1 first stage of several, R = pad
 //        period, outputs in the padded layout; 2 second stage after a MODE 1
 //        stage (Ns = pad period), inputs in the padded layout
-template <int R, bool INV, int MODE, int CN = 0, int CNS = 0, int CLSM = 1, bool MH = true>
+// TC  : compressed table (CqPlan::tc): tw[k] = w_{Ns R}^k only
+template <int R, bool INV, int MODE, int CN = 0, int CNS = 0, int CLSM = 1, bool MH = true, bool TC = false>
 __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int to,
                                       const float2 CQ_GLOBAL *__restrict__ mh)
 {
@@ -313,10 +319,20 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
             for (int r = 0; r < R; r++) v[b][r] = src[r * rs];
             if (twiddle) {
                 const float2 *twk = tw + k;
+                if constexpr (TC) {
+                    // w^r = w^(r/2) w^(r - r/2): at most log2(R) products deep
+                    float2 wp[R];
+                    wp[1] = twk[0];
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    const float2 w = twk[(r - 1) * Ns];
-                    v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+                    for (int r = 2; r < R; r++) wp[r] = cmul(wp[r >> 1], wp[r - (r >> 1)]);
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[b][r] = INV ? cmulc(v[b][r], wp[r]) : cmul(v[b][r], wp[r]);
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        const float2 w = twk[(r - 1) * Ns];
+                        v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+                    }
                 }
             }
             Dft<R, INV>::run(v[b]);
@@ -342,31 +358,38 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     __syncthreads();
 }
 
-#define CQ_STAGE_CASES(INV, MODE)                                                   \
-    case 2: cq_stage<2, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 3: cq_stage<3, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 4: cq_stage<4, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 5: cq_stage<5, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 7: cq_stage<7, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 8: cq_stage<8, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 9: cq_stage<9, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;               \
-    case 11: cq_stage<11, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
-    case 13: cq_stage<13, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
-    case 16: cq_stage<16, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
-    case 17: cq_stage<17, INV, MODE>(bo, LS, n, L, Ns, to, mh); break;             \
+#define CQ_STAGE_CASES(INV, MODE, TC)                                                           \
+    case 2: cq_stage<2, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 3: cq_stage<3, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 4: cq_stage<4, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 5: cq_stage<5, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 7: cq_stage<7, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 8: cq_stage<8, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 9: cq_stage<9, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
+    case 11: cq_stage<11, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;       \
+    case 13: cq_stage<13, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;       \
+    case 16: cq_stage<16, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;       \
+    case 17: cq_stage<17, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;       \
     default: break;
 
+// tc: this is the last stage of a plan with a compressed table
 template <bool INV>
 __device__ __forceinline__ void cq_stage_rt(int R, int mode, int bo, int LS, int n, int L, int Ns, int to,
-                                            const float2 CQ_GLOBAL *__restrict__ mh)
+                                            const float2 CQ_GLOBAL *__restrict__ mh, bool tc)
 {
     if (mode == 1) {
         if (R == 16) cq_stage<16, INV, 1>(bo, LS, n, L, Ns, to, mh);
         else cq_stage<8, INV, 1>(bo, LS, n, L, Ns, to, mh);
+    } else if (tc) {
+        if (mode == 2) {
+            switch (R) { CQ_STAGE_CASES(INV, 2, true) }
+        } else {
+            switch (R) { CQ_STAGE_CASES(INV, 0, true) }
+        }
     } else if (mode == 2) {
-        switch (R) { CQ_STAGE_CASES(INV, 2) }
+        switch (R) { CQ_STAGE_CASES(INV, 2, false) }
     } else {
-        switch (R) { CQ_STAGE_CASES(INV, 0) }
+        switch (R) { CQ_STAGE_CASES(INV, 0, false) }
     }
 }
 
@@ -385,13 +408,16 @@ __device__ __forceinline__ int cq_first_mode(const CqPlan &p)
 // The stages R, RS... of a plan of length N compiled for that length, lines
 // LSM cq_line_stride(N) apart; MODE, NS and TO are the current stage's layout
 // mode, Ns and twiddle offset (the generic loop's run-time values).
-template <bool INV, int N, int LSM, int MODE, int NS, int TO, int R, int... RS>
+// TCL: the last stage reads a compressed table (CqPlan::tc).
+template <bool INV, int N, int LSM, int MODE, int NS, int TO, bool TCL, int R, int... RS>
 __device__ __forceinline__ void cq_fft_stages(int bo, int L, int to, const float2 CQ_GLOBAL *__restrict__ mh)
 {
     constexpr int LS = LSM * cq_line_stride(N);
     if constexpr (sizeof...(RS) > 0) {
         cq_stage<R, INV, MODE, N, NS, LSM>(bo, LS, N, L, NS, to + TO, nullptr);
-        cq_fft_stages<INV, N, LSM, (MODE == 1 ? 2 : 0), NS * R, TO + (R - 1) * NS, RS...>(bo, L, to, mh);
+        cq_fft_stages<INV, N, LSM, (MODE == 1 ? 2 : 0), NS * R, TO + (R - 1) * NS, TCL, RS...>(bo, L, to, mh);
+    } else if constexpr (TCL) {
+        cq_stage<R, INV, MODE, N, NS, LSM, true, true>(bo, LS, N, L, NS, to + TO, mh);
     } else if constexpr (!INV && R == 17) {
         // compiled for the length with its multiplier code, a forward
         // radix-17 last stage needs 120 VGPRs and spills: with a multiplier
@@ -406,8 +432,8 @@ __device__ __forceinline__ void cq_fft_stages(int bo, int L, int to, const float
 
 // A plan R0 RS... of length N as compiled stages, if p is that plan and its
 // lines are LSM line strides apart; the first stage's mode follows
-// cq_first_mode.
-template <bool INV, int N, int LSM, int R0, int... RS>
+// cq_first_mode.  TCOK: also compiled with a compressed last-stage table.
+template <bool INV, bool TCOK, int N, int LSM, int R0, int... RS>
 __device__ __forceinline__ bool cq_fft_plan(const CqPlan &p, int LS, int bo, int L, int to,
                                             const float2 CQ_GLOBAL *__restrict__ mh)
 {
@@ -418,7 +444,14 @@ __device__ __forceinline__ bool cq_fft_plan(const CqPlan &p, int LS, int bo, int
     for (int s = 0; s < ns; s++)
         if (p.rad[s] != rad[s]) return false;
     constexpr int mode0 = ns > 1 && R0 == cq_pad_period(N) ? 1 : 0;
-    cq_fft_stages<INV, N, LSM, mode0, 1, 0, R0, RS...>(bo, L, to, mh);
+    if (p.tc) {
+        if constexpr (TCOK) {
+            cq_fft_stages<INV, N, LSM, mode0, 1, 0, true, R0, RS...>(bo, L, to, mh);
+            return true;
+        }
+        return false;
+    }
+    cq_fft_stages<INV, N, LSM, mode0, 1, 0, false, R0, RS...>(bo, L, to, mh);
     return true;
 }
 
@@ -430,12 +463,12 @@ template <bool INV>
 __device__ __forceinline__ bool cq_fft_fixed(const CqPlan &p, int LS, int bo, int L, int to,
                                              const float2 CQ_GLOBAL *__restrict__ mh)
 {
-    return cq_fft_plan<INV, 680, 1, 8, 5, 17>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, 680, 2, 8, 5, 17>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, 1296, 1, 16, 9, 9>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, 880, 1, 16, 5, 11>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, 880, 2, 16, 5, 11>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, 2100, 1, 4, 3, 5, 5, 7>(p, LS, bo, L, to, mh);
+    return cq_fft_plan<INV, false, 680, 1, 8, 5, 17>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, false, 680, 2, 8, 5, 17>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, false, 1296, 1, 16, 9, 9>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, false, 880, 1, 16, 5, 11>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, false, 880, 2, 16, 5, 11>(p, LS, bo, L, to, mh) ||
+           cq_fft_plan<INV, true, 2100, 1, 4, 3, 5, 5, 7>(p, LS, bo, L, to, mh);
 }
 
 // The stages of plan p as run-time stages (any plan), multiplier on the last.
@@ -446,7 +479,8 @@ __device__ __forceinline__ void cq_fft_generic(const CqPlan &p, int bo, int LS, 
     int Ns = 1, mode = cq_first_mode(p);
     for (int s = 0; s < p.ns; s++) {
         const int R = p.rad[s];
-        cq_stage_rt<INV>(R, mode, bo, LS, p.n, L, Ns, to, (s == p.ns - 1) ? mh : nullptr);
+        const bool last = s == p.ns - 1;
+        cq_stage_rt<INV>(R, mode, bo, LS, p.n, L, Ns, to, last ? mh : nullptr, last && p.tc);
         to += (R - 1) * Ns;
         Ns *= R;
         mode = mode == 1 ? 2 : 0;
