@@ -304,6 +304,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="the engine's own tiles instead of the autotuner's")
     ap.add_argument("--accuracy-steps", type=int, default=24)
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -352,8 +353,20 @@ def main():
     cells = wl["cells_per_step"]
     default_workload = args.workload == "batch128" and args.kinds == "mixed" and args.shots == 128
     ctx = engine.Context(0)
-    b = ctx.batch(jobs)
-    print("batch built: %d shots, %.2f GB on device" % (len(jobs), b.nbytes() / 1e9),
+    tiles, tune = None, None
+    if not args.no_tune:
+        # tile shapes and FFT plan from the autotuner (timed trials, cached on
+        # disk by device and window set); outside the timed region
+        cache = os.environ.get("CQ_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "cq_pstd",
+                                                                 "tiles.txt")
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        t0 = time.perf_counter()
+        tiles, tms, hit = ctx.autotune(jobs, cache)
+        tune = {"tiles": tiles, "trial_ms_per_step": round(tms, 4), "from_cache": hit,
+                "seconds": round(time.perf_counter() - t0, 1)}
+        print("autotune:", json.dumps(tune), file=sys.stderr, flush=True)
+    b = ctx.batch(jobs, tiles=tiles)
+    print("batch built: %d shots, %.2f GB on device, tiles %s" % (len(jobs), b.nbytes() / 1e9, b.tiles()),
           file=sys.stderr, flush=True)
     b.step(args.warmup)
     b.sync()
@@ -446,6 +459,7 @@ def main():
                      "step_bytes": step_bytes,
                      "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},
         "cpu_baseline": cpu,
+        "autotune": tune,
         "serve": serve,
         "qrtm": qrtm,
     }

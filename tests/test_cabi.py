@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "constant-q-pstd_amd")
 
 HEADERS = {"cq.h": "libcq_hip.so", "cq_host.h": "libcq_host.so", "cq_revolve.h": "libcq_host.so",
-           "cq_lowrank.h": "libcq_host.so"}
+           "cq_lowrank.h": "libcq_host.so", "cq_shim.h": "libcq_shim.so"}
 
 
 def declared(header):

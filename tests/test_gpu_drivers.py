@@ -243,6 +243,57 @@ def test_shipped_default_run(shipped, mode, kind, prefix, shots):
         assert e <= 1e-4
 
 
+def test_shim_aps_equals_driver(shipped, tmp_path):
+    """cq_shim_aps (libcq_shim.so, the body of the reference's aps acforward,
+    sgpstd-qmodeling/acps4.cu:184-750, for shots given as arrays) on three
+    shots of the shipped run writes reca<shot>_<nr>.bin files equal, bit for
+    bit, to those of bin/aps (same window, same tiles), and snapshots with
+    flagout = 1 (50 frames of the 301 x 160 interior for 500 steps)."""
+    import ctypes as C
+    t, vp, rho, q = shipped
+    shots = [0, 40, 79]
+    tracenum = np.fromfile(t / "geometry" / "tracenum.bin", np.int32)
+    shotxp = np.fromfile(t / "geometry" / "shotxp.bin", np.float32)
+    shotzp = np.fromfile(t / "geometry" / "shotzp.bin", np.float32)
+    if not all((t / "obsdata" / ("reca%d_%d.bin" % (s, int(tracenum[s])))).exists() for s in shots):
+        r = subprocess.run([os.path.join(BIN, "aps")], cwd=t / "work", capture_output=True, text=True, timeout=280)
+        assert r.returncode == 0, r.stdout + r.stderr
+    gxp = np.fromfile(t / "geometry" / "gxp0.bin", np.float32)
+    gzp = np.fromfile(t / "geometry" / "gzp0.bin", np.float32)
+    j = synth.make_job(0, vp, rho, 398, 160, 10.0, 10.0, 0.001, SHIP_NT, 30.0, float(shotxp[0]), float(shotzp[0]),
+                       gxp, gzp, 10, sroffmx=1500.0, want_k2=False)
+    bbx, bbz = host.bbx_bbz(398, 160, np.ascontiguousarray(rho, np.float32).ravel())
+    lib = C.CDLL(os.path.join(ROOT, "constant-q-pstd_amd", "libcq_shim.so"))
+    keep = {}
+
+    def arr(name, a, dt=np.float32):
+        keep[name] = np.ascontiguousarray(a, dt)
+        ct = C.c_float if dt == np.float32 else C.c_int
+        return keep[name].ctypes.data_as(C.POINTER(ct))
+
+    for flagout, out in ((0, tmp_path / "o0"), (1, tmp_path / "o1")):
+        out.mkdir()
+        print("case shim flagout", flagout, flush=True)
+        rc = lib.cq_shim_aps(
+            C.c_int(SHIP_NT), C.c_float(0.001), arr("rik", j["rik"]), C.c_int(len(shots)),
+            arr("ids", shots, np.int32), arr("sx", shotxp[shots]), arr("sz", shotzp[shots]),
+            C.c_int(j["nx"]), C.c_int(j["nz"]), C.c_int(j["nxx"]), C.c_int(j["nzz"]), C.c_int(10),
+            C.c_float(10.0), C.c_float(10.0), C.c_float(1500.0), C.c_float(0.0),
+            arr("vpt", vp), arr("rhot", rho), arr("bbxt", bbx), arr("bbzt", bbz), C.c_int(398), C.c_int(160),
+            arr("tn", tracenum, np.int32), C.c_int(flagout),
+            *[arr(k, j[k]) for k in ("iax", "ibx", "iaz", "ibz", "hax", "hbx", "haz", "hbz",
+                                     "kxsmr", "kxscr", "kzsmr", "kzscr", "kxsmi", "kxsci", "kzsmi", "kzsci")],
+            str(t / "geometry" / "gxp").encode(), str(t / "geometry" / "gzp").encode(), str(out).encode())
+        assert rc == 0, rc
+        for s in shots:
+            name = "reca%d_%d.bin" % (s, int(tracenum[s]))
+            assert (out / name).read_bytes() == (t / "obsdata" / name).read_bytes(), name
+        if flagout:
+            snap = np.fromfile(out / ("snapa%d_%d_%d.bin" % (shots[1], j["nz"], j["nx"])), np.float32)
+            assert snap.size == (SHIP_NT + 9) // 10 * j["nx"] * j["nz"]
+            assert np.isfinite(snap).all() and np.abs(snap).max() > 0
+
+
 @pytest.mark.parametrize("split,total", [("1,2", 2), ("2,1", 3)])
 def test_reference_shot_split(tree, split, total):
     """CQ_SHOT_SPLIT="numprocs,GPU_N" runs the shots the reference's MPI split
