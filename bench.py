@@ -50,8 +50,9 @@ def algorithmic_bytes(jobs):
       C2 reads 3 half spectra, div, gam, y2, c, writes total (20 B/cell) and
         1 half spectrum;
       R2 reads 1, writes 1 half spectrum;
-      C3 reads 2 half spectra, p, total, 3 coefficients, vz, dt*bz, writes p,
-        vz, dvz (40 B/cell), 1 half spectrum + z-CPML strips;
+      C3 reads 2 half spectra, p, total, 2 coefficients (rho v^2 dt lam and
+        dt tao / 2; dt tao is twice the latter), vz, dt*bz, writes p, vz, dvz
+        (36 B/cell), 1 half spectrum + z-CPML strips;
       R3 reads p, vx, dt*bx, dvz, writes vx, div (24 B/cell) + x-CPML strips."""
     out = {}
 
@@ -71,7 +72,7 @@ def algorithmic_bytes(jobs):
             add("k_q_r1", 6 * hs)
             add("k_q_c2", 20 * n + 4 * hs)
             add("k_q_r2", 2 * hs)
-            add("k_q_c3", 40 * n + 3 * hs + sz)
+            add("k_q_c3", 36 * n + 3 * hs + sz)
             add("k_q_r3", 24 * n + sx)
     return out
 

@@ -748,9 +748,9 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     const size_t o_ckpt = A.take<char>(sl.bytes * (size_t)snaps);
     const size_t o_bx = A.take<float>(N), o_bz = A.take<float>(N);
     const size_t o_f2 = A.take<float>(N), o_f3 = A.take<float>(N);
-    size_t o_q[4] = {0, 0, 0, 0}, o_tot = 0, o_scr = 0, o_s1 = 0, o_s3 = 0, o_s4 = 0;
+    size_t o_q[3] = {0, 0, 0}, o_tot = 0, o_scr = 0, o_s1 = 0, o_s3 = 0, o_s4 = 0;
     if (q) {
-        for (int i = 0; i < 4; i++) o_q[i] = A.take<float>(N);
+        for (int i = 0; i < 3; i++) o_q[i] = A.take<float>(N);
         o_tot = A.take<float>(N);
         o_scr = A.take<float>(N);
         o_s1 = A.take<float2>((size_t)cq_hs_cols(nxx) * ldk);
@@ -821,8 +821,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         base.c1 = f2; base.c2 = f3;
     } else {
         base.qa = f2; base.qb = f3;
-        base.qe = (float *)(mem + o_q[0]); base.qg = (float *)(mem + o_q[1]);
-        base.qy2 = (float *)(mem + o_q[2]); base.qcc = (float *)(mem + o_q[3]);
+        base.qg = (float *)(mem + o_q[0]);
+        base.qy2 = (float *)(mem + o_q[1]); base.qcc = (float *)(mem + o_q[2]);
         base.tot = (float *)(mem + o_tot);
         base.scr = (float *)(mem + o_scr);
         base.s1 =(float2 *)(mem + o_s1); base.s3 = (float2 *)(mem + o_s3); base.s4 = (float2 *)(mem + o_s4);
@@ -879,9 +879,9 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     // per-cell coefficients (field layout, zero padding)
     const float dt = j.dt;
     std::vector<float> hbx(N, 0.f), hbz(N, 0.f), hf2(N, 0.f), hf3(N, 0.f);
-    std::vector<float> hq[4];
+    std::vector<float> hq[3];
     if (q)
-        for (int i = 0; i < 4; i++) hq[i].assign(N, 0.f);
+        for (int i = 0; i < 3; i++) hq[i].assign(N, 0.f);
     for (int ix = 0; ix < nxx; ix++)
         for (int iz = 0; iz < nzz; iz++) {
             const size_t a = (size_t)ix * nzz + iz, g = cq_q4(nxx, ix, iz >> 2) + (iz & 3);
@@ -894,16 +894,19 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
                 hf2[g] = r;
                 hf3[g] = (float)((double)r * (1.0 / 12.0) * v * dt * v * dt);
             } else {
-                // acsglw.cu:1528-1529: p += (rho v^2 dt lam)(total - (dt tao/2) T) - (dt tao) aux
+                // acsglw.cu:1528-1529: p += (rho v^2 dt lam)(total - (dt tao/2) T) - (dt tao) aux.
+                // dt tao is not stored: the float dt * tao is exactly twice
+                // (float)(0.5 dt tao) (the double product of two floats is
+                // exact, and halving commutes with rounding), so the pass
+                // forms it as 2 qb bit for bit
                 const float gm = j.gam[a], lam = j.lam[a], tao = j.tao[a], vwd = j.vwd[a];
                 hf2[g] = j.rho[a] * v * v * dt * lam;
                 hf3[g] = (float)(0.5 * dt * tao);
-                hq[0][g] = dt * tao;
                 // sum_total (acsglw.cu:1232-1240): y2 = 32 gam vwd, c = dt^2 v^2 lam / 12
-                hq[1][g] = gm;
-                hq[2][g] = (float)(32.0 * gm * vwd);
+                hq[0][g] = gm;
+                hq[1][g] = (float)(32.0 * gm * vwd);
                 const float dtf = dt * dt, vpf = v * v;
-                hq[3][g] = (float)(1.0 / 12.0 * dtf * vpf * lam);
+                hq[2][g] = (float)(1.0 / 12.0 * dtf * vpf * lam);
             }
         }
     std::vector<float> h1x(4 * nxx), h1z(4 * nzz);
@@ -956,7 +959,7 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     HIPCHK(hipMemcpyAsync(f2, hf2.data(), 4 * N, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(f3, hf3.data(), 4 * N, hipMemcpyHostToDevice, s));
     if (q)
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 3; i++)
             HIPCHK(hipMemcpyAsync(mem + o_q[i], hq[i].data(), 4 * N, hipMemcpyHostToDevice, s));
     std::vector<float> hlw;
     if (lr) {

@@ -56,7 +56,7 @@ __host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
     /* per-cell coefficients; acoustic: rho v^2 dt, rho v^4 dt^3/12 */                            \
     const float G *c1; const float G *c2; const float G *dtbx; const float G *dtbz;               \
     /* viscoacoustic (see build_job) */                                                           \
-    const float G *qa; const float G *qb; const float G *qe;                                      \
+    const float G *qa; const float G *qb;                                                         \
     const float G *qg; const float G *qy2; const float G *qcc;                                    \
     /* CPML memory variables, strips only */                                                      \
     float G *psix; float G *phix; float G *psiz; float G *phiz;                                   \

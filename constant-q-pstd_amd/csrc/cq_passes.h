@@ -1302,7 +1302,7 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
     const float src = rcv ? 0.f : J.rik[t] * J.src_scale;
     const float s = J.qsign;
     struct C3In {
-        float4 p, tot, qa, qb, qe, tt;
+        float4 p, tot, qa, qb, tt;
     };
     cq_loop<2, true>(
         CW * nq,
@@ -1315,7 +1315,6 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
             r.tot = ld4(J.tot + g);
             r.qa = ld4(J.qa + g);
             r.qb = ld4(J.qb + g);
-            r.qe = ld4(J.qe + g);
             r.tt = ld4c(J.scr + g);
             return r;
         },
@@ -1332,8 +1331,10 @@ __global__ __launch_bounds__(CQ_NT, CQ_WAVES_PER_SIMD) void k_q_c3(const JobD *_
             for (int k = 0; k < 4; k++) {
                 const int iz = 4 * zq + k;
                 if (iz >= nzz) continue;
-                float v = f4get(r.p, k) + f4get(r.qa, k) * (f4get(r.tot, k) - s * f4get(r.qb, k) * f4get(r.tt, k)) -
-                          s * f4get(r.qe, k) * line[2 * iz];
+                // dt tao = 2 qb exactly (see build_job)
+                const float qb = f4get(r.qb, k);
+                float v = f4get(r.p, k) + f4get(r.qa, k) * (f4get(r.tot, k) - s * qb * f4get(r.tt, k)) -
+                          s * (2.0f * qb) * line[2 * iz];
                 if (!rcv && ix == J.xsg && iz == J.zsg) v += src;
                 line[2 * iz] = v;
             }
