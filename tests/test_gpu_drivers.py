@@ -243,6 +243,73 @@ def test_shipped_default_run(shipped, mode, kind, prefix, shots):
         assert e <= 1e-4
 
 
+@pytest.mark.parametrize("mode", ["aps", "vaps", "artm", "qrtm"])
+def test_shims_equal_drivers(tree, tmp_path, mode):
+    """The four acforward bodies of libcq_shim.so (cq_shim_aps / vaps / artm /
+    qrtm; sgpstd-qmodeling/acps4.cu:184-198, acsglw.cu:185-200,
+    sgpstd-qrtm/acps4.cu:183-198, acsglw.cu:189-204) on the three shots of the
+    generated dataset: the modeling shims write gather files equal, bit for
+    bit, to bin/aps's / bin/vaps's; the RTM shims add to img_sp / ssg_sp
+    exactly the stack bin/artm / bin/qrtm write (shots summed in the same
+    order, every shot's image independent of its batch)."""
+    import ctypes as C
+    t, vp, rho, q = tree
+    q_kind = mode in ("vaps", "qrtm")
+    rtm = mode in ("artm", "qrtm")
+    if rtm:
+        run(t, "vaps" if q_kind else "aps")   # observed data
+    run(t, mode)
+    n = len(SHOTS)
+    tracenum = np.fromfile(t / "geometry" / "tracenum.bin", np.int32)
+    sx = np.fromfile(t / "geometry" / "shotxp.bin", np.float32)
+    sz = np.fromfile(t / "geometry" / "shotzp.bin", np.float32)
+    j = shot_job(1 if q_kind else 0, vp, rho, q, SHOTS[0])
+    bbx, bbz = host.bbx_bbz(NXT, NZT, np.ascontiguousarray(rho, np.float32).ravel())
+    keep = []
+
+    def F(a):
+        keep.append(np.ascontiguousarray(a, np.float32))
+        return keep[-1].ctypes.data_as(C.POINTER(C.c_float))
+
+    def I(a):
+        keep.append(np.ascontiguousarray(a, np.int32))
+        return keep[-1].ctypes.data_as(C.POINTER(C.c_int))
+
+    head = [C.c_int(NT), C.c_float(DT_MS / 1000.0), F(j["rik"]), C.c_int(n), I(np.arange(n)), F(sx), F(sz),
+            C.c_int(j["nx"]), C.c_int(j["nz"]), C.c_int(j["nxx"]), C.c_int(j["nzz"]), C.c_int(PML),
+            C.c_float(DX), C.c_float(DX), C.c_float(j["sroffmx"]), C.c_float(0.0), F(vp)]
+    if q_kind:
+        tq = host.transfermodel(FD, FO, np.ascontiguousarray(vp, np.float32).ravel(),
+                                np.ascontiguousarray(q, np.float32).ravel())
+        head += [F(tq["vwd"]), F(tq["tao"]), F(tq["lam"]), F(tq["gam"])]
+    head += [F(rho), F(bbx), F(bbz), C.c_int(NXT), C.c_int(NZT), I(tracenum)]
+    coef = [F(j[k]) for k in ("iax", "ibx", "iaz", "ibz", "hax", "hbx", "haz", "hbz",
+                              "kxsmr", "kxscr", "kzsmr", "kzscr", "kxsmi", "kxsci", "kzsmi", "kzsci")]
+    names = [str(t / "geometry" / "gxp").encode(), str(t / "geometry" / "gzp").encode()]
+    lib = C.CDLL(os.path.join(ROOT, "constant-q-pstd_amd", "libcq_shim.so"))
+    print("case shim", mode, flush=True)
+    if not rtm:
+        rc = getattr(lib, "cq_shim_" + mode)(*head, C.c_int(0), *coef, *names, str(tmp_path).encode())
+        assert rc == 0, rc
+        prefix = "recva" if q_kind else "reca"
+        for i in range(n):
+            name = "%s%d_%d.bin" % (prefix, i, int(tracenum[i]))
+            assert (tmp_path / name).read_bytes() == (t / "obsdata" / name).read_bytes(), name
+        return
+    img = np.zeros(NXT * NZT, np.float32)
+    ssg = np.zeros(NXT * NZT, np.float32)
+    rtm_tail = [img.ctypes.data_as(C.POINTER(C.c_float)), ssg.ctypes.data_as(C.POINTER(C.c_float)),
+                C.c_float(FD), C.c_float(1500.0)]
+    if q_kind:
+        rtm_tail += [C.c_float(1e-20), C.c_float(1e-20)]
+    recname = str(t / "obsdata" / ("recva" if q_kind else "reca")).encode()
+    rc = getattr(lib, "cq_shim_" + mode)(C.c_int(12), *head, *coef, *rtm_tail, *names, recname)
+    assert rc == 0, rc
+    got = np.fromfile(t / "work" / ("QRTM.BIN" if q_kind else "RTM.BIN"), np.float32)
+    assert np.abs(img).max() > 0
+    assert np.array_equal(got[:NXT * NZT], img) and np.array_equal(got[NXT * NZT:], ssg)
+
+
 def test_shim_aps_equals_driver(shipped, tmp_path):
     """cq_shim_aps (libcq_shim.so, the body of the reference's aps acforward,
     sgpstd-qmodeling/acps4.cu:184-750, for shots given as arrays) on three
