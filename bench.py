@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="the engine's own tiles instead of the autotuner's")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="global steps per captured HIP graph (0: direct launches)")
     ap.add_argument("--accuracy-steps", type=int, default=24)
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -367,6 +369,7 @@ def main():
                 "seconds": round(time.perf_counter() - t0, 1)}
         print("autotune:", json.dumps(tune), file=sys.stderr, flush=True)
     b = ctx.batch(jobs, tiles=tiles)
+    b.use_graphs(args.graph_steps)
     print("batch built: %d shots, %.2f GB on device, tiles %s" % (len(jobs), b.nbytes() / 1e9, b.tiles()),
           file=sys.stderr, flush=True)
     b.step(args.warmup)
@@ -461,6 +464,7 @@ def main():
                      "step_GBs": round(step_bytes * args.steps / el / 1e9, 1)},
         "cpu_baseline": cpu,
         "autotune": tune,
+        "graph_steps": args.graph_steps,
         "serve": serve,
         "qrtm": qrtm,
     }

@@ -62,7 +62,8 @@ SYMBOLS = ["cq_open", "cq_close", "cq_last_error", "cq_version", "cq_batch_creat
            "cq_copy_calibrate", "cq_batch_kernel_times", "cq_pass_name", "cq_batch_read_image",
            "cq_server_open", "cq_server_submit", "cq_server_advance", "cq_server_poll",
            "cq_server_job_steps", "cq_server_result", "cq_server_release", "cq_server_close",
-           "cq_forward_static", "cq_batch_create_tiled", "cq_batch_get_tiles", "cq_autotune"]
+           "cq_forward_static", "cq_batch_create_tiled", "cq_batch_get_tiles", "cq_autotune",
+           "cq_batch_use_graphs"]
 
 _lib = None
 
@@ -209,6 +210,10 @@ class Batch:
             t = Tiles.of(tiles)
             _chk(lib().cq_batch_create_tiled(ctx.h, arr, C.c_int(len(jobs)), C.byref(t), C.byref(self.h)))
         ctx.children.add(self)
+
+    def use_graphs(self, steps_per_graph):
+        """cq_batch_use_graphs: run spans of that many steps as HIP graphs (0: off)."""
+        _chk(lib().cq_batch_use_graphs(self.h, C.c_int(steps_per_graph)))
 
     def tiles(self):
         t = Tiles()

@@ -555,8 +555,25 @@ struct cq_batch_s {
     // alone (no side streams or events).
     char *pool = nullptr;
     size_t pool_cap = 0, pool_off = 0;
+    // Graph-captured steps (cq_batch_use_graphs): graph_k global steps per
+    // graph, one executable graph per combination of active roles; each
+    // launch group's kernels read the global step from its counter dstep[g]
+    // (launched with step -1), which the graph advances.  dstep_t: the step
+    // the counters hold (-1: unknown).
+    int graph_k = 0;
+    bool dry = false;   // launch() only raises the LDS limits (before a capture)
+    int *dstep = nullptr;
+    int dstep_t = -1;
+    std::map<int, hipGraphExec_t> graphs;
+    void drop_graphs()
+    {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
     ~cq_batch_s()
     {
+        drop_graphs();
+        if (!pool && dstep) (void)hipFree(dstep);
         if (!pool) {
             for (auto &j : jobs) (void)hipFree(j.mem);
             (void)hipFree(djobs);
@@ -1170,6 +1187,18 @@ static int rebuild_tables(cq_batch b)
             HIPCHK(hipMalloc(&b->djobs, sizeof(JobD) * 2 * b->djobs_cap));
         }
     }
+    b->drop_graphs();
+    b->dstep_t = -1;   // groups may have changed: the counters are set again before a graph runs
+    if (!b->dstep) {
+        if (b->pool) {
+            const int rc = pool_take(b, sizeof(int) * kGroups, (void **)&b->dstep);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMalloc(&b->dstep, sizeof(int) * kGroups));
+        }
+        HIPCHK(hipMemsetAsync(b->dstep, 0, sizeof(int) * kGroups, s));
+        b->dstep_t = -1;
+    }
     const int cap = (int)b->djobs_cap;
     std::vector<JobD> hd(2 * (size_t)cap);
     memset(hd.data(), 0, sizeof(JobD) * hd.size());
@@ -1183,6 +1212,7 @@ static int rebuild_tables(cq_batch b)
         b->tmax = std::max(b->tmax, h.t0 + h.nsched);
         const int g = is_q(h.kind) ? (nq++ & 1) : 2 + (na++ & 1);
         h.grp = g;
+        hd[i].dstep = hd[cap + i].dstep = b->dstep + g;
         for (int r = 0; r < h.nroles; r++) {
             const int e = r ? cap + (int)i : (int)i;
             std::vector<int2> *it = items[g][r];
@@ -1348,6 +1378,7 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
 {
     const size_t lds = b->lds[pass];
     HIPCHK(raise_lds_limit(b->ctx->device, reinterpret_cast<const void *>(kernel), lds));
+    if (b->dry) return CQ_OK;
     // one launch per group on its stream, or, unsplit, one for all groups
     for (int g = 0; g < kGroups; g++) {
         const int o = b->split ? b->goff[role][pass][g] : 0;
@@ -1411,14 +1442,38 @@ static int launch_pass(cq_batch b, int pass, int role, int step)
 // and role, for the roles that advance at this step (a modeling shot's one
 // wavefield while it runs; an RTM shot's source or receiver wavefield as its
 // schedule says).
+// Bit r of the result: some wavefield of role r advances at global step t.
+static int active_roles(cq_batch b, int t)
+{
+    int m = 0;
+    for (const JobH &h : b->jobs) {
+        const int s = t - h.t0;
+        if (s < 0 || s >= h.nsched) continue;
+        m |= 1 << (h.hsched.empty() ? 0 : h.hsched[s].x);
+    }
+    return m;
+}
+
+// The launches of one global step for the roles in `mask`; `step` is passed
+// to the kernels (-1 in a captured graph: each group's counter).
+static int step_launches(cq_batch b, int step, int mask)
+{
+    for (int r = 0; r < 2; r++) {
+        if (!(mask >> r & 1)) continue;
+        for (int p = 0; p < P_COUNT; p++) {
+            if (b->nitems[r][p] == 0) continue;
+            int rc = launch_pass(b, p, r, step);
+            if (rc) return rc;
+        }
+    }
+    return CQ_OK;
+}
+
 static int one_step(cq_batch b, int step)
 {
-    bool active[2] = {false, false};
-    for (const JobH &h : b->jobs) {
-        const int s = step - h.t0;
-        if (s < 0 || s >= h.nsched) continue;
-        active[h.hsched.empty() ? 0 : h.hsched[s].x] = true;
-    }
+    const int mask = active_roles(b, step);
+    const bool active[2] = {(mask & 1) != 0, (mask & 2) != 0};
+    if (!b->timing) return step_launches(b, step, mask);
     if (b->timing) {
         hipEvent_t e0;
         HIPCHK(hipEventCreate(&e0));
@@ -1475,9 +1530,8 @@ static int ckpt_apply(cq_batch b, const JobH &h, const CkptOp &op)
 // Start (fork) or end (join) the side streams of launch groups 1 ..
 // kGroups - 1.  Per-launch timing keeps one stream so that its events
 // bracket single passes.
-static int fork_side(cq_batch b)
+static int make_side(cq_batch b)
 {
-    if (b->timing || b->pool) return CQ_OK;
     if (!b->fork_ev) {
         HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
         for (int g = 1; g < kGroups; g++) {
@@ -1485,6 +1539,13 @@ static int fork_side(cq_batch b)
             HIPCHK(hipEventCreateWithFlags(&b->join_ev[g], hipEventDisableTiming));
         }
     }
+    return CQ_OK;
+}
+static int fork_side(cq_batch b)
+{
+    if (b->timing || b->pool) return CQ_OK;
+    const int rc = make_side(b);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(b->fork_ev, b->ctx->stream));
     for (int g = 1; g < kGroups; g++) HIPCHK(hipStreamWaitEvent(b->side[g], b->fork_ev, 0));
     b->split = true;
@@ -1501,9 +1562,85 @@ static int join_side(cq_batch b)
     return CQ_OK;
 }
 
+__global__ void k_step_inc(int *c) { c[0] += 1; }
+__global__ void k_set_steps(int *c, int v)
+{
+    if (threadIdx.x < kGroups) c[threadIdx.x] = v;
+}
+
+// The executable graph of graph_k global steps for the roles in `mask`
+// (captured on first use): fork to the group streams, per step each group's
+// passes (step -1: the group's counter) and its counter increment, join.
+// Null (and graphs switched off, with the reason in the last error) if the
+// runtime refuses the capture.
+static hipGraphExec_t step_graph(cq_batch b, int mask)
+{
+    auto it = b->graphs.find(mask);
+    if (it != b->graphs.end()) return it->second;
+    hipStream_t s = b->ctx->stream;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    // streams, events and the passes' LDS limits are set up outside the capture
+    int rc = make_side(b);
+    if (rc == CQ_OK) {
+        b->dry = true;
+        rc = step_launches(b, -1, mask);
+        b->dry = false;
+    }
+    if (rc) return nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        set_err(CQ_EDEVICE, "hipStreamBeginCapture failed");
+        return nullptr;
+    }
+    bool used[kGroups] = {};
+    for (const JobH &h : b->jobs) used[h.grp] = true;
+    rc = fork_side(b);
+    for (int k = 0; rc == CQ_OK && k < b->graph_k; k++) {
+        rc = step_launches(b, -1, mask);
+        for (int gi = 0; rc == CQ_OK && gi < kGroups; gi++) {
+            if (!used[gi]) continue;   // no job reads this counter
+            hipStream_t st = gi > 0 ? b->side[gi] : s;
+            hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, st, b->dstep + gi);
+            if (hipGetLastError() != hipSuccess) rc = set_err(CQ_EDEVICE, "step counter launch failed");
+        }
+    }
+    if (rc == CQ_OK) rc = join_side(b);
+    b->split = false;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc == CQ_OK && e == hipSuccess && g) {
+        if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (!ex) {
+        if (rc == CQ_OK) set_err(CQ_EDEVICE, "graph capture of the step launches failed");
+        b->graph_k = 0;
+        return nullptr;
+    }
+    b->graphs[mask] = ex;
+    return ex;
+}
+
+// Whether the global steps t .. t + k - 1 can run as one graph: no
+// checkpoint copy inside them (after their first step) and the same roles
+// active throughout.
+static bool graph_span(cq_batch b, int t, int k, int &mask)
+{
+    mask = active_roles(b, t);
+    if (!mask) return false;
+    for (int u = t + 1; u < t + k; u++)
+        if (active_roles(b, u) != mask) return false;
+    for (const JobH &h : b->jobs)
+        for (size_t o = h.op_next; o < h.ops.size(); o++) {
+            const int at = h.ops[o].at + h.t0;
+            if (at >= t + k) break;
+            if (at > t) return false;
+        }
+    return true;
+}
+
 static int advance_steps(cq_batch b, int end)
 {
-    for (; b->t < end; b->t++) {
+    while (b->t < end) {
         for (auto &h : b->jobs) {
             const int s = b->t - h.t0;
             while (h.op_next < h.ops.size() && h.ops[h.op_next].at <= s) {
@@ -1514,12 +1651,32 @@ static int advance_steps(cq_batch b, int end)
                 h.op_next++;
             }
         }
+        int mask = 0;
+        if (b->graph_k > 0 && !b->timing && !b->pool && end - b->t >= b->graph_k &&
+            graph_span(b, b->t, b->graph_k, mask)) {
+            // the graph orders after the context stream only: the groups'
+            // direct launches join it first (and before any capture)
+            int rc = join_side(b);
+            if (rc) return rc;
+            hipGraphExec_t ex = step_graph(b, mask);
+            if (ex) {
+                if (b->dstep_t != b->t) {
+                    hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, b->ctx->stream, b->dstep, b->t);
+                    HIPCHK(hipGetLastError());
+                }
+                HIPCHK(hipGraphLaunch(ex, b->ctx->stream));
+                b->t += b->graph_k;
+                b->dstep_t = b->t;
+                continue;
+            }
+        }
         if (!b->split) {
             int rc = fork_side(b);
             if (rc) return rc;
         }
         int rc = one_step(b, b->t);
         if (rc) return rc;
+        b->t++;
     }
     return CQ_OK;
 }
@@ -2028,6 +2185,17 @@ static void destroy_children(cq_ctx ctx)
 {
     while (!ctx->servers.empty()) cq_server_close(*ctx->servers.begin());
     while (!ctx->batches.empty()) cq_batch_destroy(*ctx->batches.begin());
+}
+
+extern "C" int cq_batch_use_graphs(cq_batch b, int steps_per_graph)
+{
+    if (!b || steps_per_graph < 0 || steps_per_graph > 64)
+        return set_err(CQ_EINVAL, "cq_batch_use_graphs: steps_per_graph %d outside 0..64", steps_per_graph);
+    if (b->pool) return set_err(CQ_EINVAL, "cq_batch_use_graphs: not for the static scratch batch");
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    b->drop_graphs();
+    b->graph_k = steps_per_graph;
+    return CQ_OK;
 }
 
 extern "C" int cq_batch_enable_timing(cq_batch b, int on)

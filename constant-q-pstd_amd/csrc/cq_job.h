@@ -40,6 +40,9 @@ __host__ __device__ inline int cq_hs_cols(int nxx) { return (nxx + 7) & ~7; }
     /* {role, t, t_aux, flags} of its step s (RTM), or none: t = s, t_aux = s - 1 (modeling) */   \
     int t0, nsched;                                                                               \
     const int4 G *sched;                                                                          \
+    /* the launch group's global step counter, read when a pass is launched with step < 0 */      \
+    /* (graph-captured steps) */                                                                  \
+    const int G *dstep;                                                                           \
     CqPlan px, pz;                                                                                \
     const float2 G *twx;                                                                          \
     const float2 G *twz;                                                                          \

@@ -684,12 +684,15 @@ __device__ __forceinline__ bool cq_step_of(const JobG &J, int step, int &t, int 
 // make it reload the record's fields (scalar loads behind a wait for every
 // outstanding store) in each loop iteration after a store.  The FFT plans,
 // indexed by stage at run time, are read through the reference (Jr).
+// A negative step argument (a launch recorded in a graph that is replayed
+// for many steps) means: the global step is the launch group's counter.
 #define CQ_ITEM_PROLOGUE                                             \
     const int2 it = items[blockIdx.x];                               \
     if (it.x < 0) return;                                            \
     const JobG &Jr = reinterpret_cast<const JobG *>(jobs)[it.x];     \
     const JobG J = Jr;                                               \
     int t, t_aux, flags;                                             \
+    if (step < 0) step = *J.dstep;                                   \
     if (!cq_step_of(J, step, t, t_aux, flags)) return;               \
     (void)t_aux;                                                     \
     (void)flags;

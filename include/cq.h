@@ -228,6 +228,17 @@ int cq_batch_get_tiles(cq_batch b, cq_tiles *out);
 int cq_autotune(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const char *cache, cq_tiles *out,
                 double *tuned_ms, int *from_cache);
 
+/* Graph-captured steps (no counterpart in the reference, which launches
+ * every kernel of every step from the host, sgpstd-qmodeling/acps4.cu:508-645).
+ * With steps_per_graph = k > 0, cq_batch_step / cq_batch_run run spans of k
+ * global steps as one HIP graph (the four launch groups' pass chains and
+ * their step counters, captured once per combination of active RTM roles and
+ * replayed), wherever the span holds no checkpoint copy and the same roles
+ * throughout; other steps, and all steps while per-pass timing is on, are
+ * launched directly.  Results are identical to direct launches bit for bit.
+ * 0 switches graphs off (the default).  k at most 64. */
+int cq_batch_use_graphs(cq_batch b, int steps_per_graph);
+
 /* Per-pass device timing with HIP events on the engine's stream.  While
  * enabled, every launch of cq_batch_step is bracketed by events; passes are
  * numbered 0..CQ_NPASS-1 (cq_pass_name gives the kernel name): acoustic row,
