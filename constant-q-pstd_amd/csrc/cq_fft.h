@@ -65,8 +65,15 @@ struct CqPlan {
 // (16 when 16 | n, else 8 when 8 | n), value i sits at slot i + i / P.  The
 // first stage's outputs, R apart in value order, then land R + 1 apart (no
 // bank conflicts), and both stages still address their R values at one
-// constant stride.
-__host__ __device__ constexpr int cq_pad_period(int n) { return n % 16 == 0 ? 16 : n % 8 == 0 ? 8 : 0; }
+// constant stride.  Lines longer than CQ_PAD_MAX points are not padded: there
+// the n / P extra slots per line (4725 instead of 4201 slots at n = 4200)
+// would push two-line tiles past 80 KB and halve the workgroups per CU, which
+// costs more than the first stage's bank conflicts.
+#define CQ_PAD_MAX 2048
+__host__ __device__ constexpr int cq_pad_period(int n)
+{
+    return n > CQ_PAD_MAX ? 0 : n % 16 == 0 ? 16 : n % 8 == 0 ? 8 : 0;
+}
 __host__ __device__ constexpr int cq_line_stride(int n)
 {
     return (n + (cq_pad_period(n) ? n / cq_pad_period(n) : 0)) | 1;
@@ -294,7 +301,9 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     const int total = L * m;
     // exact quotients by the hardware reciprocal (1 ulp): operands are below
     // 2^14, so (i + 0.5) / d lies at least 0.5 / d from an integer while the
-    // product is off by less than 2^-8 / d
+    // product is off by less than 2^-8 / d.  Every index product below has
+    // operands under 2^23: a full-rate 24-bit multiply (__mul24), not the
+    // quarter-rate 32-bit one
     const float inv_m = __builtin_amdgcn_rcpf((float)m), inv_ns = __builtin_amdgcn_rcpf((float)Ns);
     // slots between a butterfly's inputs: m, or m + m / Ns in the padded
     // layout (the period Ns divides m)
@@ -312,9 +321,9 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
         kk[b] = 0;
         if (idx < total) {
             const int l = (int)(((float)idx + 0.5f) * inv_m);
-            const int j = idx - l * m;
-            const int k = j - (int)(((float)j + 0.5f) * inv_ns) * Ns;
-            const float2 *src = buf + l * LS + (MODE == 2 ? j + (j >> psh) : j);
+            const int j = idx - __mul24(l, m);
+            const int k = j - __mul24((int)(((float)j + 0.5f) * inv_ns), Ns);
+            const float2 *src = buf + __mul24(l, LS) + (MODE == 2 ? j + (j >> psh) : j);
 #pragma unroll
             for (int r = 0; r < R; r++) v[b][r] = src[r * rs];
             if (twiddle) {
@@ -338,8 +347,8 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
             Dft<R, INV>::run(v[b]);
             // output q is value kk + q Ns: slot kk + q Ns, or, padded (Ns = 1,
             // k = 0), j (R + 1) + q
-            kk[b] = (j - k) * R + k;
-            base[b] = l * LS + (MODE == 1 ? j * (R + 1) : kk[b]);
+            kk[b] = __mul24(j - k, R) + k;
+            base[b] = __mul24(l, LS) + (MODE == 1 ? __mul24(j, R + 1) : kk[b]);
             if (MH && mh != nullptr) {
 #pragma unroll
                 for (int q = 0; q < R; q++) v[b][q] = cmul(v[b][q], mh[kk[b] + q * Ns]);
