@@ -143,13 +143,47 @@ struct cq_ctx_s {
     }
 };
 
+// Fewest odd radices (each of per-stage capacity >= need) whose product is
+// o, ties broken by the smaller radix sum (fewer registers per butterfly);
+// radices are tried largest first, so `cur` stays non-increasing.
+static void odd_split(int o, int need, int from, std::vector<int> &cur, std::vector<int> &best)
+{
+    static const int kOdd[] = {27, 25, 21, 17, 15, 13, 11, 9, 7, 5, 3};
+    auto sum = [](const std::vector<int> &v) {
+        int t = 0;
+        for (int r : v) t += r;
+        return t;
+    };
+    if (o == 1) {
+        if (best.empty() || cur.size() < best.size() || (cur.size() == best.size() && sum(cur) < sum(best)))
+            best = cur;
+        return;
+    }
+    if (!best.empty() && cur.size() >= best.size()) return;   // needs one more radix: no better
+    for (int i = from; i < (int)(sizeof kOdd / sizeof kOdd[0]); i++) {
+        const int r = kOdd[i];
+        if (o % r != 0 || cq_stage_capacity(r) < need) continue;
+        cur.push_back(r);
+        odd_split(o / r, need, i, cur, best);
+        cur.pop_back();
+    }
+}
+
 // Factorisation of n into supported radices for a workgroup that transforms
-// `need` points at once.  Variant 0: the fewest stages, taking powers of two
-// as 16/8/4/2 and powers of three as 9/3, then 5, 7, 11, 13, 17; variant 1:
-// radices of at most 8 (powers of two as 8/4/2, powers of three as 3), more
-// stages of fewer registers each.  Only radices whose per-stage capacity
-// (cq_stage_capacity) holds `need` points are taken.  False if n has another
-// prime factor or a prime factor's radix is too small.
+// `need` points at once.  Variant 0: the fewest stages: powers of two as
+// 16/8/4/2, the odd part in the fewest odd radices (odd_split: 3 ... 17 and
+// the composites 15, 21, 25, 27).  Order: a length with a pad period
+// (cq_pad_period) starts with its largest power-of-two radix (the padded
+// first-stage layout), then come the composites (largest first), the other
+// powers of two, and the odd primes and 9 (smallest first), so that the last
+// stage, which may carry a multiplier or a compressed twiddle table, is never
+// a composite (those are compiled without either); a length whose odd radices
+// are all composites and that has no power of two to end on takes the plain
+// greedy factorisation below.  Variant 1: radices of at most 8 (powers of two
+// as 8/4/2, powers of three as 3), more stages of fewer registers each.
+// Only radices whose per-stage capacity (cq_stage_capacity) holds `need`
+// points are taken.  False if n has another prime factor or a prime factor's
+// radix is too small.
 static bool make_direct_plan(int n, int need, CqPlan &p, int variant = 0)
 {
     memset(&p, 0, sizeof p);
@@ -161,6 +195,33 @@ static bool make_direct_plan(int n, int need, CqPlan &p, int variant = 0)
         m /= r;
         return true;
     };
+    if (variant != 1) {
+        std::vector<int> pow2;
+        for (int r : {16, 8, 4, 2}) {
+            if (cq_stage_capacity(r) < need) continue;
+            while (m % r == 0) {
+                pow2.push_back(r);
+                m /= r;
+            }
+        }
+        std::vector<int> cur, odd;
+        odd_split(m, need, 0, cur, odd);
+        if (m != 1 && odd.empty()) return false;
+        m = n;
+        auto composite = [](int r) { return r == 15 || r == 21 || r == 25 || r == 27; };
+        size_t i2 = 0;
+        if (cq_pad_period(n) && !pow2.empty() && pow2[0] == cq_pad_period(n) && !push(pow2[i2++])) return false;
+        for (int r : odd)
+            if (composite(r) && !push(r)) return false;
+        for (; i2 < pow2.size(); i2++)
+            if (!push(pow2[i2])) return false;
+        for (auto it = odd.rbegin(); it != odd.rend(); ++it)
+            if (!composite(*it) && !push(*it)) return false;
+        if (m == 1 && (p.ns == 0 || !composite(p.rad[p.ns - 1]))) return true;
+        memset(&p, 0, sizeof p);
+        p.n = n;
+        m = n;
+    }
     static const int kOrder[2][11] = {{16, 8, 4, 2, 9, 3, 5, 7, 11, 13, 17}, {8, 4, 2, 3, 5, 7, 11, 13, 17, 0, 0}};
     for (int r : kOrder[variant == 1 ? 1 : 0]) {
         if (r == 0) break;

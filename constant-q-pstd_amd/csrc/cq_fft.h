@@ -6,7 +6,10 @@
 // (w_n^j = exp(-2 pi i j / n), generated in double on the host), runs an
 // in-register DFT_R with compile-time constants, then writes the autosorted
 // outputs back to the same buffer after a barrier.  Supported radices:
-// 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17 (any length whose factors are these).
+// 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17 and the composite odd radices 15, 21,
+// 25, 27 (one Cooley-Tukey split in registers), so any length whose prime
+// factors are at most 17; the composites take the 3/5/7-rich lengths in fewer
+// stages, i.e. fewer passes through LDS.
 // A complex multiplier can be fused into the last forward stage's store.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -36,7 +39,7 @@
 // butterflies per thread per stage for radix R
 __host__ __device__ constexpr int cq_nb(int R)
 {
-    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : 1;
+    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : R == 15 ? 2 : 1;
 }
 
 // A transform plan: stages of radices rad[0 .. ns) for n points.  A length
@@ -256,6 +259,45 @@ template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
 template <bool INV> struct Dft<17, INV> : DftOdd<17, INV> {};
 
+// Composite radix R = R1 R2 by one Cooley-Tukey split in registers: input
+// x[R2 n1 + n2]; DFT_R1 over n1 for each n2, times w_R^(n2 k1), DFT_R2 over
+// n2 for each k1 gives X[k1 + R1 k2].  The inner results replace the inputs
+// they are computed from, so about R complex values are live at a time.
+template <int R1, int R2, bool INV>
+struct DftCT {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        constexpr int R = R1 * R2;
+        constexpr CqRoots<R> T{};
+        float2 y[R2][R1];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) {
+            float2 t[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) t[n1] = v[R2 * n1 + n2];
+            Dft<R1, INV>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; k1++) {
+                const int e = (n2 * k1) % R;
+                y[n2][k1] = e == 0 ? t[k1] : cmul(t[k1], make_float2(T.c[e], INV ? T.s[e] : -T.s[e]));
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) {
+            float2 t[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) t[n2] = y[n2][k1];
+            Dft<R2, INV>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = t[k2];
+        }
+    }
+};
+template <bool INV> struct Dft<15, INV> : DftCT<3, 5, INV> {};
+template <bool INV> struct Dft<21, INV> : DftCT<3, 7, INV> {};
+template <bool INV> struct Dft<25, INV> : DftCT<5, 5, INV> {};
+template <bool INV> struct Dft<27, INV> : DftCT<3, 9, INV> {};
+
 // Dynamic LDS of every kernel.  The out-of-line stage functions address it
 // through this symbol (by float2 offsets) so that their accesses compile to
 // LDS instructions rather than generic (flat) ones.
@@ -367,6 +409,17 @@ __device__ __noinline__ void cq_stage(int bo, int LS, int n, int L, int Ns, int 
     __syncthreads();
 }
 
+// Composite radices hold R values and their sub-DFTs' temporaries, so they
+// are compiled without the multiplier code and the compressed-table form:
+// plans never make a composite the last stage (make_direct_plan), which is
+// the only stage that carries a multiplier or a compressed table.  With
+// those forms they would spill to scratch, and a kernel with any callee
+// that uses scratch sets up scratch for every wave it launches.
+#define CQ_STAGE_COMPOSITE(INV, MODE)                                                          \
+    case 15: cq_stage<15, INV, MODE, 0, 0, 1, false>(bo, LS, n, L, Ns, to, nullptr); break;    \
+    case 21: cq_stage<21, INV, MODE, 0, 0, 1, false>(bo, LS, n, L, Ns, to, nullptr); break;    \
+    case 25: cq_stage<25, INV, MODE, 0, 0, 1, false>(bo, LS, n, L, Ns, to, nullptr); break;    \
+    case 27: cq_stage<27, INV, MODE, 0, 0, 1, false>(bo, LS, n, L, Ns, to, nullptr); break;
 #define CQ_STAGE_CASES(INV, MODE, TC)                                                           \
     case 2: cq_stage<2, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
     case 3: cq_stage<3, INV, MODE, 0, 0, 1, true, TC>(bo, LS, n, L, Ns, to, mh); break;         \
@@ -396,9 +449,9 @@ __device__ __forceinline__ void cq_stage_rt(int R, int mode, int bo, int LS, int
             switch (R) { CQ_STAGE_CASES(INV, 0, true) }
         }
     } else if (mode == 2) {
-        switch (R) { CQ_STAGE_CASES(INV, 2, false) }
+        switch (R) { CQ_STAGE_COMPOSITE(INV, 2) CQ_STAGE_CASES(INV, 2, false) }
     } else {
-        switch (R) { CQ_STAGE_CASES(INV, 0, false) }
+        switch (R) { CQ_STAGE_COMPOSITE(INV, 0) CQ_STAGE_CASES(INV, 0, false) }
     }
 }
 
@@ -465,7 +518,7 @@ __device__ __forceinline__ bool cq_fft_plan(const CqPlan &p, int LS, int bo, int
 }
 
 // Lengths with compiled plans: the windows of the batched workload (680 deep,
-// 1296 wide) and of the Q-RTM workload (880 deep, 2100 wide); lines one or,
+// 1296 wide) and of the Q-RTM workload (880 deep, 2100 = 25 21 4 wide); lines one or,
 // for k_q_c2's interleaved lines, two line strides apart.  True if the
 // transform ran.
 template <bool INV>
@@ -477,10 +530,13 @@ __device__ __forceinline__ bool cq_fft_fixed(const CqPlan &p, int LS, int bo, in
            cq_fft_plan<INV, false, 1296, 1, 16, 9, 9>(p, LS, bo, L, to, mh) ||
            cq_fft_plan<INV, false, 880, 1, 16, 5, 11>(p, LS, bo, L, to, mh) ||
            cq_fft_plan<INV, false, 880, 2, 16, 5, 11>(p, LS, bo, L, to, mh) ||
-           cq_fft_plan<INV, true, 2100, 1, 4, 3, 5, 5, 7>(p, LS, bo, L, to, mh);
+           cq_fft_plan<INV, true, 2100, 1, 25, 21, 4>(p, LS, bo, L, to, mh);
 }
 
 // The stages of plan p as run-time stages (any plan), multiplier on the last.
+// Inlined: out of line it would need a stack frame (the loop's state lives
+// across the stage calls), i.e. scratch for every wave of every pass, and the
+// measured row passes ran 12% slower that way.
 template <bool INV>
 __device__ __forceinline__ void cq_fft_generic(const CqPlan &p, int bo, int LS, int L, int to,
                                                const float2 CQ_GLOBAL *__restrict__ mh)
