@@ -167,6 +167,19 @@ def fft_friendly(n, primes=(2, 3, 5, 7)):
         m += 1
 
 
+def natural_size(job, nxx, nzz):
+    """Record the padded window a request asked for (nxx_natural x
+    nzz_natural cells, CPML included) beside the FFT-friendly window it is
+    computed on: throughput counts the requested cells, not the widening."""
+    job["nxx_natural"], job["nzz_natural"] = int(nxx), int(nzz)
+    return job
+
+
+def useful_cells(job):
+    """Cells of the window a request asked for (see natural_size)."""
+    return job.get("nxx_natural", job["nxx"]) * job.get("nzz_natural", job["nzz"])
+
+
 def make_window_job(kind, vpt, rhot, nxt, nzt, dx, dz, dt, nt, fd, left_col, nx, src_col,
                     src_z, rec_cols, rec_z, pml, qt=None, fo=None, flagsource=2, want_k2=False):
     """A shot on an explicit window of model columns [left_col, left_col+nx)
@@ -224,13 +237,14 @@ def batch128_jobs(nt=5000, nshots=128, kinds=None, want_k2=False, shots=None, wi
         c = first + 18 * i
         left = max(0, c - 600)
         right = min(nxt - 1, c + 600)
-        nx = right - left + 1
+        nx = nx0 = right - left + 1
         if widen:
             nx = fft_friendly(nx + 2 * pml) - 2 * pml
         kind = 0 if kinds is None else kinds[i]
         jobs.append(make_window_job(kind, vp, rho, nxt, nzt, dx, dx, 0.0004, nt, 20.0, left, nx, c,
                                     2, np.arange(left, right + 1), 2, pml, qt=q, fo=20.0,
                                     want_k2=want_k2))
+        natural_size(jobs[-1], nx0 + 2 * pml, 600 + 2 * pml)
     return jobs
 
 
@@ -256,8 +270,10 @@ def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 4096), nz_range
         r = np.random.default_rng(seed * 1000 + k)
         u = r.uniform()
         kind = SERVE_CLASSES[0][0] if u < 0.5 else (SERVE_CLASSES[1][0] if u < 0.8 else SERVE_CLASSES[2][0])
-        nx = fft_friendly(int(r.integers(nx_range[0], nx_range[1] + 1)) + 2 * pml) - 2 * pml
-        nz = fft_friendly(int(r.integers(nz_range[0], nz_range[1] + 1)) + 2 * pml) - 2 * pml
+        nx0 = int(r.integers(nx_range[0], nx_range[1] + 1))
+        nz0 = int(r.integers(nz_range[0], nz_range[1] + 1))
+        nx = fft_friendly(nx0 + 2 * pml) - 2 * pml
+        nz = fft_friendly(nz0 + 2 * pml) - 2 * pml
         nt = int(r.integers(nt_range[0], nt_range[1] + 1))
         dx = float(np.round(r.uniform(*dx_range) * 4) / 4)
         fd = float(np.round(r.uniform(*fd_range)))
@@ -267,6 +283,7 @@ def serve_jobs(rate=40.0, window_s=1.0, seed=501, nx_range=(512, 4096), nz_range
                                    qrange=(30.0, 300.0))
         job = make_window_job(kind, vp, rho, nx, nz, dx, dx, dt, nt, fd, 0, nx, nx // 2, 2,
                               np.arange(nx), 2, pml, qt=q, fo=fd)
+        natural_size(job, nx0 + 2 * pml, nz0 + 2 * pml)
         if kind == 3:
             job["obs"] = r.standard_normal((nt, len(job["gxg"]))).astype(np.float32)
             job["snaps"] = snaps
@@ -303,7 +320,7 @@ def qrtm_jobs(nshots=32, nt=6000, snaps=40, kind=3, seed=401, obs_seed=402, wide
         job["obs"] = rng.standard_normal((nt, len(job["gxg"]))).astype(np.float32)
         job["snaps"] = snaps
         job["etas"] = job["etar"] = 1e-20
-        jobs.append(job)
+        jobs.append(natural_size(job, 2001 + 2 * pml, nzt + 2 * pml))
     return jobs
 
 
