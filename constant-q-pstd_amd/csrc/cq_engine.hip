@@ -44,7 +44,7 @@
 #include "cq_passes.h"
 #include "cq_revolve.h"
 
-#define CQ_VERSION "cq-pstd-mi355x 0.2"
+#define CQ_VERSION "cq-pstd-mi355x 0.3"
 
 // ---------------------------------------------------------------------------
 // errors
