@@ -202,8 +202,9 @@ int cq_forward_static(cq_ctx ctx, const cq_job_desc *job, float *rec);
  *        k_q_c3, k_lr_c2): the same values
  *   tk   kz rows per tile of k_q_r1 and k_lr_r1: 1, 2, 4 or 8
  *   tk2  kz rows per tile of k_q_r2: 1, 2, 3, 4, 6, 8, 12 or 16
- *   plan FFT factorisation: 0 largest radices first (powers of two as 16,
- *        powers of three as 9), 1 radices of at most 8 (no 16 or 9)
+ *   plan FFT factorisation: 0 the fewest stages (powers of two as 16, odd
+ *        parts in the fewest of 3..17 and the composites 15, 21, 25, 27),
+ *        1 radices of at most 8 (no 16, 9 or composites)
  * A field of 0 (plan: -1) leaves that choice to the engine. */
 typedef struct {
     int tz, cw, cwq, tk, tk2, plan;
