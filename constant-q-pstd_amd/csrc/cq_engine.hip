@@ -2337,6 +2337,19 @@ extern "C" int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *
     return CQ_OK;
 }
 
+extern "C" int cq_fft_plan(int n, int points, int variant, int *radices, int cap, int *inner)
+{
+    if (n < 1 || points < n || variant < 0 || variant > 1 || (cap > 0 && !radices) || cap < 0)
+        return set_err(CQ_EINVAL, "cq_fft_plan: bad arguments");
+    CqPlan p;
+    if (!make_plan(n, points, p, variant))
+        return set_err(CQ_EUNSUPPORTED, "no FFT plan for %d points (%d per workgroup, variant %d)", n, points,
+                       variant);
+    for (int s = 0; s < p.ns && s < cap; s++) radices[s] = p.rad[s];
+    if (inner) *inner = p.bn ? p.n : 0;
+    return p.ns;
+}
+
 extern "C" int cq_plain_forward(cq_ctx ctx, const cq_job_desc *job, float *rec)
 {
     if (!ctx || !job || !rec) return set_err(CQ_EINVAL, "cq_plain_forward: bad arguments");

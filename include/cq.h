@@ -258,6 +258,16 @@ int cq_batch_pass_times(cq_batch b, double *row_ms, double *col_ms, int *count);
  * byte counters and achievable bandwidth. */
 int cq_copy_calibrate(cq_ctx ctx, long long bytes, int reps, double *ms_per_copy);
 
+/* The FFT plan the engine takes for an n-point transform of which one
+ * workgroup holds `points` points per stage (lines x n), in plan variant 0 or
+ * 1 (cq_tiles::plan), for inspection and tests; needs no GPU (no counterpart
+ * in the reference, whose cuFFT plans are opaque).  Writes the stages'
+ * radices, first to last, to radices[0 .. cap) and returns their count;
+ * *inner is 0 for a direct transform, or, for a length with a prime factor
+ * above 17, the length M of the Bluestein transform's inner transforms (the
+ * radices are then M's).  CQ_EUNSUPPORTED when no plan holds `points`. */
+int cq_fft_plan(int n, int points, int variant, int *radices, int cap, int *inner);
+
 #ifdef __cplusplus
 }
 #endif
