@@ -2256,6 +2256,13 @@ extern "C" int cq_batch_use_graphs(cq_batch b, int steps_per_graph)
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     b->drop_graphs();
     b->graph_k = steps_per_graph;
+    // capture the graph of the roles active now (capture and instantiation
+    // take milliseconds: done here, the first steps that replay it do not
+    // pay for them); other role combinations are captured when first met
+    if (b->graph_k > 0 && !b->jobs.empty()) {
+        const int mask = active_roles(b, b->t);
+        if (mask && !step_graph(b, mask) && b->graph_k == 0) return CQ_EDEVICE;
+    }
     return CQ_OK;
 }
 

@@ -237,7 +237,10 @@ int cq_autotune(cq_ctx ctx, const cq_job_desc *jobs, int njobs, const char *cach
  * replayed), wherever the span holds no checkpoint copy and the same roles
  * throughout; other steps, and all steps while per-pass timing is on, are
  * launched directly.  Results are identical to direct launches bit for bit.
- * 0 switches graphs off (the default).  k at most 64. */
+ * The graph of the roles active at the batch's current step is captured by
+ * this call; other combinations when first met.  0 switches graphs off (the
+ * default).  k at most 64.  CQ_EDEVICE if the capture failed (graphs are then
+ * off). */
 int cq_batch_use_graphs(cq_batch b, int steps_per_graph);
 
 /* Per-pass device timing with HIP events on the engine's stream.  While
