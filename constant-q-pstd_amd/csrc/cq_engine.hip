@@ -273,7 +273,10 @@ static bool make_plan(int n, int need, CqPlan &p, int variant = 0)
         p.bn = n;
     }
     p.tc = p.n > kTcMin && p.ns > 1 ? 1 : 0;
-    return true;
+    // the device compiles composite stages only as non-last stages (no
+    // multiplier, no compressed table); make_direct_plan never ends on one
+    const int last = p.rad[p.ns - 1];
+    return !(last == 15 || last == 21 || last == 25 || last == 27);
 }
 
 // Entries of a plan's twiddle table: n (n - 1 used), or 2 n / R - 1 with a
