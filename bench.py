@@ -16,7 +16,9 @@ pass once over every shot) with its measured traffic,
 the FP32 reference-faithful CPU path (the oracle's FP32 build: 9 full 2-D C2C
 FFTs per step, OpenMP) timed on a bounded sample of the same workload, the
 serving phase (continuous batching of Poisson arrivals: throughput, p50/p99
-latency) and the Q-RTM phase (32 revolve-checkpointed shots).  Cell-steps
+latency; shortest remaining work first within a run set of 32 M work cells,
+and beside it every resident job advancing at every step) and the Q-RTM
+phase (32 revolve-checkpointed shots).  Cell-steps
 count the windows the shots ask for (CPML included); the engine computes on
 windows widened to FFT-friendly sizes (about 1% more cells in the batched
 workload), reported beside as computed_mcell_steps_per_s.
@@ -204,7 +206,7 @@ def profiled_traffic(kernel):
     return None
 
 
-def serve_bench(rate, window_s, seed, chunk):
+def serve_bench(rate, window_s, seed, chunk, run_cells=0):
     """Serving workload (Poisson arrivals of mixed acoustic / viscoacoustic /
     Q-RTM requests, each with its own window, nt, dx and wavelet) through the
     engine's continuous-batching server.  Requests are generated before the
@@ -224,7 +226,7 @@ def serve_bench(rate, window_s, seed, chunk):
             useful += c * j["nt"]
             executed += cp * j["nt"]
     ctx = engine.Context(0)
-    srv = ctx.server(mx, mz)
+    srv = ctx.server(mx, mz, run_cells=run_cells)
     pending = list(reqs)
     info, lat = {}, {}
     t0 = time.perf_counter()
@@ -265,7 +267,7 @@ def serve_bench(rate, window_s, seed, chunk):
             "useful_mcell_steps_per_s": round(useful / (wall - first) / 1e6, 1),
             "executed_mcell_steps_per_s": round(executed / (wall - first) / 1e6, 1),
             "p50_s": pct(allv, 50), "p99_s": pct(allv, 99), "per_class": per,
-            "steps_per_advance": chunk}
+            "steps_per_advance": chunk, "run_cells": run_cells}
 
 
 def qrtm_bench(nshots, nt, snaps):
@@ -332,6 +334,11 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-window", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-run-cells", type=int, default=32000000,
+                    help="server run set in work cells: the jobs with the least remaining work "
+                         "advance (0: every resident job advances every step)")
+    ap.add_argument("--no-serve-all", action="store_true",
+                    help="skip the second serving run with every resident job advancing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one "
@@ -433,8 +440,14 @@ def main():
     serve = None
     if not args.no_serve:
         print("serving phase", file=sys.stderr, flush=True)
-        serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk)
+        serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, args.serve_run_cells)
         print("serve:", json.dumps(serve), file=sys.stderr, flush=True)
+        if args.serve_run_cells > 0 and not args.no_serve_all:
+            # the same arrivals with every resident job advancing at every
+            # global step (throughput-first policy), for comparison
+            print("serving phase, all resident jobs advancing", file=sys.stderr, flush=True)
+            serve["all_resident"] = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, 0)
+            print("serve all resident:", json.dumps(serve["all_resident"]), file=sys.stderr, flush=True)
 
     qrtm = None
     if not args.no_qrtm:

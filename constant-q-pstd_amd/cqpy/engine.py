@@ -35,7 +35,7 @@ class JobDesc(C.Structure):
 
 class ServerDesc(C.Structure):
     _fields_ = [("max_nxx", C.c_int), ("max_nzz", C.c_int), ("hbm_budget", C.c_longlong),
-                ("max_jobs", C.c_int)]
+                ("max_jobs", C.c_int), ("run_cells", C.c_longlong)]
 
 
 class Tiles(C.Structure):
@@ -185,8 +185,8 @@ class Context:
                                C.byref(t), C.byref(ms), C.byref(hit)))
         return t.as_dict(), ms.value, bool(hit.value)
 
-    def server(self, max_nxx, max_nzz, hbm_budget=0, max_jobs=0):
-        return Server(self, max_nxx, max_nzz, hbm_budget, max_jobs)
+    def server(self, max_nxx, max_nzz, hbm_budget=0, max_jobs=0, run_cells=0):
+        return Server(self, max_nxx, max_nzz, hbm_budget, max_jobs, run_cells)
 
     def copy_calibrate(self, nbytes, reps=10):
         """ms per streaming copy of nbytes (reads nbytes, writes nbytes)."""
@@ -293,10 +293,10 @@ class Batch:
 class Server:
     """Continuous-batching server (cq_server_* of include/cq.h)."""
 
-    def __init__(self, ctx, max_nxx, max_nzz, hbm_budget=0, max_jobs=0):
+    def __init__(self, ctx, max_nxx, max_nzz, hbm_budget=0, max_jobs=0, run_cells=0):
         self.ctx = ctx
         self.jobs = {}
-        d = ServerDesc(int(max_nxx), int(max_nzz), int(hbm_budget), int(max_jobs))
+        d = ServerDesc(int(max_nxx), int(max_nzz), int(hbm_budget), int(max_jobs), int(run_cells))
         self.h = C.c_void_p()
         _chk(lib().cq_server_open(ctx.h, C.byref(d), C.byref(self.h)))
         ctx.children.add(self)

@@ -1888,7 +1888,15 @@ struct cq_server_s {
     int *dflag = nullptr;           // device flags of that check, one per wavefield
     size_t dflag_cap = 0;
     std::set<int> fail_admit;       // CQ_FAULT_ADMIT: ids whose admission is made to fail
-    ~cq_server_s() { (void)hipFree(dflag); }
+    // run_cells > 0: resident jobs outside the run set, their ids and the
+    // steps each had taken when it stopped (its device memory stays)
+    std::vector<JobH> paused;
+    std::vector<int> paused_ids, paused_done;
+    ~cq_server_s()
+    {
+        (void)hipFree(dflag);
+        for (auto &h : paused) (void)hipFree(h.mem);
+    }
 };
 
 // Global steps between two checks of the running jobs' wavefields for
@@ -2061,6 +2069,89 @@ static int check_nonfinite(cq_server s)
     return CQ_OK;
 }
 
+// Work cells of a resident job (cq_server_desc::run_cells): cells per step,
+// viscoacoustic cells counted twice (their step runs six passes to two).
+static double work_cells(const JobH &h) { return (double)h.nxx * h.nzz * (is_q(h.kind) ? 2.0 : 1.0); }
+
+// run_cells > 0: the resident jobs with the least remaining work advance, as
+// many as fit run_cells work cells (at least one); the others are paused
+// (taken out of the batch, their state kept on the device) and resume with
+// their start step moved so that their local step continues where it
+// stopped.  Sets `changed` when the batch's jobs changed; the caller
+// rebuilds the tables.  Nothing may be in flight on the stream, and the
+// running jobs' non-finite flags have been acted on (server_turnover
+// retires blown jobs before it chooses).
+static void server_reschedule(cq_server s, bool &changed)
+{
+    if (s->desc.run_cells <= 0) return;
+    cq_batch b = &s->b;
+    struct Cand {
+        bool paused;
+        size_t idx;
+        int id;
+        double work, cells;
+    };
+    std::vector<Cand> c;
+    for (size_t i = 0; i < b->jobs.size(); i++) {
+        const JobH &h = b->jobs[i];
+        c.push_back({false, i, s->running[i], (double)(h.nsched - (b->t - h.t0)) * work_cells(h), work_cells(h)});
+    }
+    for (size_t i = 0; i < s->paused.size(); i++) {
+        const JobH &h = s->paused[i];
+        c.push_back({true, i, s->paused_ids[i], (double)(h.nsched - s->paused_done[i]) * work_cells(h),
+                     work_cells(h)});
+    }
+    std::sort(c.begin(), c.end(), [](const Cand &a, const Cand &z) {
+        return a.work != z.work ? a.work < z.work : a.id < z.id;
+    });
+    std::vector<char> run(c.size(), 0);
+    double sum = 0;
+    for (size_t k = 0; k < c.size(); k++)
+        if (k == 0 || sum < (double)s->desc.run_cells) {
+            run[k] = 1;
+            sum += c[k].cells;
+        }
+    bool any = false;
+    for (size_t k = 0; k < c.size(); k++) any = any || (run[k] != 0) == c[k].paused;
+    if (!any) return;
+    // new running and paused sets, in the order chosen
+    std::vector<JobH> nj, np;
+    std::vector<int> nr, npi, npd;
+    std::vector<char> nb;
+    for (size_t k = 0; k < c.size(); k++) {
+        if (run[k]) {
+            if (c[k].paused) {
+                JobH &h = s->paused[c[k].idx];
+                h.t0 = b->t - s->paused_done[c[k].idx];
+                for (int r = 0; r < 2; r++) h.d[r].t0 = h.t0;
+                nj.push_back(std::move(h));
+                nb.push_back(0);
+            } else {
+                nj.push_back(std::move(b->jobs[c[k].idx]));
+                nb.push_back(s->blown.size() > c[k].idx ? s->blown[c[k].idx] : 0);
+            }
+            nr.push_back(c[k].id);
+        } else {
+            if (c[k].paused) {
+                npd.push_back(s->paused_done[c[k].idx]);
+                np.push_back(std::move(s->paused[c[k].idx]));
+            } else {
+                JobH &h = b->jobs[c[k].idx];
+                npd.push_back(b->t - h.t0);
+                np.push_back(std::move(h));
+            }
+            npi.push_back(c[k].id);
+        }
+    }
+    b->jobs = std::move(nj);
+    s->running = std::move(nr);
+    s->blown = std::move(nb);
+    s->paused = std::move(np);
+    s->paused_ids = std::move(npi);
+    s->paused_done = std::move(npd);
+    changed = true;
+}
+
 static void fail_job(ServerJob &sj, int err, int t)
 {
     sj.state = CQ_JOB_FAILED;
@@ -2149,6 +2240,12 @@ static int server_turnover(cq_server s)
         for (auto &v : sj.f) std::vector<float>().swap(v);   // inputs now live on the device
         changed = true;
     }
+    // run_cells > 0: the run set is chosen again at every turnover (a job
+    // finished or was admitted, a non-finite check, the end of an advance)
+    if (s->desc.run_cells > 0 && (changed || !s->paused.empty())) {
+        if (!synced) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        server_reschedule(s, changed);
+    }
     if (changed) return rebuild_tables(b);
     return CQ_OK;
 }
@@ -2173,7 +2270,7 @@ extern "C" int cq_server_advance(cq_server s, int nsteps, int *running, int *que
         }
     }
     if (rc) return rc;
-    if (running) *running = (int)b->jobs.size();
+    if (running) *running = (int)(b->jobs.size() + s->paused.size());
     if (queued) *queued = (int)s->queue.size();
     return CQ_OK;
 }

@@ -139,6 +139,14 @@ typedef struct {
     int max_nxx, max_nzz;   /* largest padded window a job may have */
     long long hbm_budget;   /* device bytes the resident jobs may hold; 0: 90% of free memory */
     int max_jobs;           /* jobs resident at once; 0: no limit */
+    /* Work cells advanced per global step (0: every resident job advances at
+     * every step, the throughput-first default).  Otherwise the resident jobs
+     * with the least remaining work (steps left x cells, viscoacoustic cells
+     * counted twice) advance, as many as fit this many work cells (at least
+     * one); the others wait in HBM and resume where they stopped, chosen
+     * again whenever a job finishes or is admitted and every 256 global
+     * steps.  A job's results do not depend on when it runs. */
+    long long run_cells;
 } cq_server_desc;
 
 /* CQ_JOB_FAILED: the job could not be admitted (e.g. its device allocation

@@ -100,3 +100,31 @@ def test_serve_rejects_oversized_and_unfinished(ctx):
     with pytest.raises(engine.CqError):
         s.release(i)
     s.close()
+
+
+def test_serve_run_cells_shortest_first_equals_alone(ctx):
+    """run_cells = 1 (one job per global step): jobs run shortest remaining
+    work first; a short job submitted later pauses the long Q-RTM job in the
+    middle of its revolve schedule, which then resumes where it stopped.
+    Every result equals the job run alone, bit for bit."""
+    js = job_mix()
+    alone = [run_alone(ctx, j) for j in js]
+    s = ctx.server(MAXX, MAXZ, run_cells=1)
+    q = s.submit(js[2])                  # Q-RTM (revolve checkpoints)
+    running, queued = s.advance(40)
+    assert running == 1 and queued == 0
+    a = s.submit(js[0])                  # shorter: pauses the Q-RTM job
+    others = [s.submit(js[k]) for k in (1, 3, 4)]
+    ids = [a, q] + others
+    while any(s.state(i) != engine.DONE for i in ids):
+        s.advance(25)
+    st = {i: s.steps(i) for i in ids}
+    for i in ids:
+        print("job", i, "steps (submitted, started, finished)", st[i], flush=True)
+    # the short acoustic job finished before the Q-RTM job it interrupted
+    assert st[a][2] < st[q][2]
+    # one job at a time: finish steps are distinct
+    assert len({st[i][2] for i in ids}) == len(ids)
+    for k, i in zip([0, 2, 1, 3, 4], ids):
+        assert same(s.result(i), alone[k]), k
+    s.close()
