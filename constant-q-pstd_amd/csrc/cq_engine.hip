@@ -275,7 +275,7 @@ static bool make_plan(int n, int need, CqPlan &p, int variant = 0)
     p.tc = p.n > kTcMin && p.ns > 1 ? 1 : 0;
     // the device compiles composite stages only as non-last stages (no
     // multiplier, no compressed table); make_direct_plan never ends on one
-    const int last = p.rad[p.ns - 1];
+    const int last = p.ns > 0 ? p.rad[p.ns - 1] : 0;
     return !(last == 15 || last == 21 || last == 25 || last == 27);
 }
 

@@ -261,8 +261,9 @@ template <bool INV> struct Dft<17, INV> : DftOdd<17, INV> {};
 
 // Composite radix R = R1 R2 by one Cooley-Tukey split in registers: input
 // x[R2 n1 + n2]; DFT_R1 over n1 for each n2, times w_R^(n2 k1), DFT_R2 over
-// n2 for each k1 gives X[k1 + R1 k2].  The inner results replace the inputs
-// they are computed from, so about R complex values are live at a time.
+// n2 for each k1 gives X[k1 + R1 k2].  Each input is dead once its inner DFT
+// has read it and each inner result once its outer DFT has, so about R
+// complex values are live at a time.
 template <int R1, int R2, bool INV>
 struct DftCT {
     __device__ __forceinline__ static void run(float2 *v)

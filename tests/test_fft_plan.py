@@ -116,7 +116,8 @@ def test_bad_arguments():
     import sys
     code = ("import ctypes as C; L = C.CDLL(%r); r = (C.c_int * 4)(); "
             "print(L.cq_fft_plan(0, 8, 0, r, 4, None) < 0, L.cq_fft_plan(64, 32, 0, r, 4, None) < 0, "
-            "L.cq_fft_plan(64, 64, 2, r, 4, None) < 0, L.cq_fft_plan(64, 64, 0, r, 4, None) > 0)"
+            "L.cq_fft_plan(64, 64, 2, r, 4, None) < 0, L.cq_fft_plan(64, 64, 0, r, 4, None) > 0, "
+            "L.cq_fft_plan(1, 1, 0, r, 4, None) == 0)"
             % os.path.join(ROOT, "constant-q-pstd_amd", "libcq_hip.so"))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
-    assert out.split() == ["True"] * 4
+    assert out.split() == ["True"] * 5
