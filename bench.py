@@ -270,10 +270,11 @@ def serve_bench(rate, window_s, seed, chunk, run_cells=0):
             "steps_per_advance": chunk, "run_cells": run_cells}
 
 
-def qrtm_bench(nshots, nt, snaps):
+def qrtm_bench(nshots, nt, snaps, graph_steps):
     """Q-RTM with revolve checkpointing on the 800 x 3000 workload (2100 x 880
     padded windows, `snaps` checkpoints per shot), all shots in one batch, run
-    to completion with a shortened time axis (nt steps).  Executed
+    to completion with a shortened time axis (nt steps; spans of graph_steps
+    steps between checkpoint copies replayed as HIP graphs).  Executed
     cell-steps count every forward and adjoint step the schedules take
     (numforw + nt per shot); useful ones are 2 nt per shot."""
     from cqpy import engine, host, synth
@@ -283,6 +284,7 @@ def qrtm_bench(nshots, nt, snaps):
     nf = host.numforw(nt, snaps)
     ctx = engine.Context(0)
     b = ctx.batch(jobs)
+    b.use_graphs(graph_steps)
     gb = b.nbytes() / 1e9
     t0 = time.perf_counter()
     b.run()
@@ -452,7 +454,7 @@ def main():
     qrtm = None
     if not args.no_qrtm:
         print("Q-RTM phase", file=sys.stderr, flush=True)
-        qrtm = qrtm_bench(args.qrtm_shots, args.qrtm_nt, 40)
+        qrtm = qrtm_bench(args.qrtm_shots, args.qrtm_nt, 40, args.graph_steps)
         print("qrtm:", json.dumps(qrtm), file=sys.stderr, flush=True)
 
     cpu = None
