@@ -25,9 +25,11 @@
 
 
 // Workgroups of 512 threads.  A thread holds the butterflies it computes in
-// one stage in registers (cq_nb(R) of them: 16-18 complex values, 32 for
+// one stage in registers (cq_nb(R) of them: 16-22 complex values, 32 for
 // radices 7/8 whose butterflies need few temporaries), which keeps every pass
 // at <= 128 VGPRs, i.e. 4 waves per SIMD when two workgroups share a CU.
+// Two radix-11 butterflies per thread let 880-point columns (16 5 11) be
+// transformed eight lines at a time; two radix-13 or radix-17 ones spill.
 #ifndef CQ_NT
 #define CQ_NT 512
 #endif
@@ -39,7 +41,7 @@
 // butterflies per thread per stage for radix R
 __host__ __device__ constexpr int cq_nb(int R)
 {
-    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : R == 15 ? 2 : 1;
+    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : R == 11 ? 2 : R == 15 ? 2 : 1;
 }
 
 // A transform plan: stages of radices rad[0 .. ns) for n points.  A length
