@@ -31,6 +31,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -255,24 +256,38 @@ static int bluestein_len(int n)
 // length.
 static int fft_len(int n) { return smooth(n, {2, 3, 5, 7, 11, 13, 17}) ? n : bluestein_len(n); }
 
+// LDS budget per workgroup that the tile choice aims for: 80 KB lets two
+// workgroups share a CU.
+#ifndef CQ_LDS_TILE
+#define CQ_LDS_TILE (80 * 1024)
+#endif
+
 // Plan of an n-point transform for `need` points (need / n lines) per
 // workgroup: direct, or Bluestein over fft_len(n) points per line.
-// Transforms longer than this keep a compressed last-stage twiddle table
-// (CqPlan::tc): for them the full table would take LDS from the lines.
+// Transforms longer than kTcMin keep a compressed last-stage twiddle table
+// (CqPlan::tc): for them the full table would take LDS from the lines.  So
+// does a shorter one whose lines and full table would exceed the tile budget
+// CQ_LDS_TILE while its lines and compressed table fit (1080 points, 8 lines:
+// 86.4 KB with the full table of 1080 entries, 79.7 KB with 239).
 static const int kTcMin = 2048;
 
 static bool make_plan(int n, int need, CqPlan &p, int variant = 0)
 {
     if (n < 1) return false;
     const int M = fft_len(n);
+    const int lines = (need + n - 1) / n;
     if (M == n) {
         if (!make_direct_plan(n, need, p, variant)) return false;
     } else {
-        const int lines = (need + n - 1) / n;
         if (!make_direct_plan(M, lines * M, p, variant)) return false;
         p.bn = n;
     }
     p.tc = p.n > kTcMin && p.ns > 1 ? 1 : 0;
+    if (!p.tc && p.ns > 1) {
+        const size_t ls = (size_t)lines * cq_line_stride(p.n);
+        const size_t full = (p.n + ls) * 8, comp = (2 * (p.n / p.rad[p.ns - 1]) - 1 + ls) * 8;
+        if (full > (size_t)CQ_LDS_TILE && comp <= (size_t)CQ_LDS_TILE) p.tc = 1;
+    }
     // the device compiles composite stages only as non-last stages (no
     // multiplier, no compressed table); make_direct_plan never ends on one
     const int last = p.ns > 0 ? p.rad[p.ns - 1] : 0;
@@ -545,8 +560,16 @@ static int rtm_schedule(int nt, int snaps, std::vector<int4> &steps, std::vector
     }
 }
 
+// Workgroup tiles of one job's passes: tz z rows per row tile, cw columns per
+// column tile (of the job's kind), tk / tk2 kz rows per k_q_r1 / k_q_r2 tile.
+struct JobTiles {
+    int tz = 0, cw = 0, tk = 0, tk2 = 0;
+};
+
 struct JobH {
     JobD d[2];              // role 0: source / modeling wavefield; role 1: receiver wavefield (RTM)
+    JobTiles tl;            // the tiles its data layout and plans are built for
+    size_t lds[CQ_NPASS] = {};   // LDS bytes of each pass with those tiles
     int nroles = 1;
     void *mem = nullptr;
     size_t bytes = 0;
@@ -566,18 +589,15 @@ struct JobH {
 
 // One launch of the per-step sequence.
 static const size_t kLdsMax = 160 * 1024;
-// LDS budget per workgroup that the tile choice aims for: 80 KB lets two
-// workgroups share a CU; tiles fall back to kLdsMax when a window needs it.
-#ifndef CQ_LDS_TILE
-#define CQ_LDS_TILE (80 * 1024)
-#endif
+// LDS budget per workgroup that the tile choice aims for (CQ_LDS_TILE);
+// tiles fall back to kLdsMax when a window needs it.
 static const size_t kLdsTile = CQ_LDS_TILE;
 
 // In launch order: a job's passes run in this order within a step (low-rank
 // jobs take LR1 / LR2 where the others take QR1 / QC2).
 enum PassId { P_AROW, P_ACOL, P_QC1, P_QR1, P_LR1, P_QC2, P_LR2, P_QR2, P_QC3, P_QR3, P_COUNT };
 static_assert(P_COUNT == CQ_NPASS, "pass count of the C ABI");
-static const int kGroups = 4;   // launch groups of a batch (see cq_batch_s::goff)
+static const int kGroups = 4;   // launch groups of a batch (see cq_batch_s::segs)
 static const char *kPassName[P_COUNT] = {"k_row_ac", "k_col_ac", "k_q_c1", "k_q_r1", "k_lr_r1",
                                          "k_q_c2",   "k_lr_c2",  "k_q_r2", "k_q_c3", "k_q_r3"};
 
@@ -606,10 +626,21 @@ struct cq_batch_s {
     // Item groups: the jobs of a batch are split into kGroups groups (two of
     // viscoacoustic jobs, two of acoustic jobs), each with its own chain of
     // launches on its own stream (group 0 on the context's stream), so that
-    // kernels of different groups overlap on the device.  Group g's items of
-    // a pass are ditems[r][p][goff[r][p][g] .. goff[r][p][g + 1]).  `split`
-    // is set while the side streams have forked from the context stream.
-    int goff[2][P_COUNT][kGroups + 1] = {};
+    // kernels of different groups overlap on the device.  A pass's items
+    // come in launch segments, one per tile width of the pass among the jobs
+    // (one unless a server job has tiles of its own, tiles_for); group g's
+    // items of segment `sg` are ditems[r][p][sg.off[g] .. sg.off[g + 1]).
+    // `split` is set while the side streams have forked from the context
+    // stream.
+    struct Seg {
+        int val = 0;       // the pass's tile width (tz, cw / cwq, tk or tk2)
+        size_t lds = 0;    // dynamic LDS of its launches
+        int off[kGroups + 1] = {};
+    };
+    std::vector<Seg> segs[2][P_COUNT];
+    // own_tiles (a server's batch): a job whose window the batch's tiles do
+    // not fit within the LDS budget takes tiles of its own (tiles_for)
+    bool own_tiles = false;
     hipStream_t side[kGroups] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kGroups] = {};
     bool split = false;
@@ -754,15 +785,19 @@ static void tile_csr(const std::vector<int> &key, int tw, int ntile, std::vector
     }
 }
 
-static bool r1_fused_for(int nxx, int tk, int variant);
-static void job_lds(cq_batch b, const JobD &d, bool q, size_t *out);
+static void job_lds(const JobTiles &tl, const JobD &d, bool q, size_t *out);
+static int tiles_for(cq_batch b, const cq_job_desc &j, JobTiles &out);
+static void job_needs(const cq_job_desc &j, const JobTiles &t, int variant, bool &fused, int &need_x, int &need_z);
+static JobTiles batch_tiles(cq_batch b, bool q);
 
-// Allocates and uploads one shot for the batch's tile shapes.  With
+// Allocates and uploads one shot for its tile shapes (tiles_for).  With
 // bytes_only != null only its device footprint is computed.
 static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_only = nullptr)
 {
     cq_ctx ctx = b->ctx;
-    const int cw = is_q(j.kind) ? b->cwq : b->cw, tk = b->tk, tk2 = b->tk2, tz = b->tz;
+    int rc = tiles_for(b, j, h.tl);
+    if (rc) return rc;
+    const int cw = h.tl.cw, tk = h.tl.tk, tk2 = h.tl.tk2, tz = h.tl.tz;
     const bool q = is_q(j.kind), rtm = is_rtm(j.kind);
     const int nxx = j.nxx, nzz = j.nzz, pml = j.pml;
     const int ldz = ((nzz + tz - 1) / tz * tz + 31) & ~31;   // whole row tiles, 128-B rows
@@ -788,13 +823,10 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.qsign = (j.kind == CQ_KIND_QRTM) ? -1.0f : 1.0f;
     base.stab = (j.kind == CQ_KIND_QRTM) ? 1 : 0;
     base.gather = rtm ? 0 : 1;
-    base.r1_fused = q && r1_fused_for(nxx, tk, b->plan) ? 1 : 0;
-    int rc;
-    // points per workgroup transform: row tiles tz/2 lines, kz-row tiles 2 tk
-    // (fused k_q_r1) or tk lines and tk2 lines (k_q_r2) of nxx; column tiles
-    // cw/2 lines of nzz
-    const int need_x = std::max((tz / 2) * nxx, q ? std::max((base.r1_fused ? 2 : 1) * tk, tk2) * nxx : 0),
-              need_z = (cw / 2) * nzz;
+    bool fused;
+    int need_x, need_z;
+    job_needs(j, h.tl, b->plan, fused, need_x, need_z);
+    base.r1_fused = fused ? 1 : 0;
     if (bytes_only) {
         // footprint only: the plans must exist, nothing is uploaded
         if (!make_plan(nxx, need_x, base.px, b->plan) || !make_plan(nzz, need_z, base.pz, b->plan))
@@ -813,12 +845,17 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
         // the passes' LDS for this window with the batch's tiles; a
         // Bluestein length may need more than the batch's widest window
         size_t need_lds[P_COUNT];
-        job_lds(b, base, q, need_lds);
+        job_lds(h.tl, base, q, need_lds);
         for (int pp = 0; pp < P_COUNT; pp++)
             if (need_lds[pp] > kLdsMax)
                 return set_err(CQ_EUNSUPPORTED, "window %dx%d: pass %s needs %zu bytes of LDS (at most %zu)", nxx, nzz,
                                kPassName[pp], need_lds[pp], kLdsMax);
-        if (!bytes_only)
+        for (int pp = 0; pp < P_COUNT; pp++) h.lds[pp] = need_lds[pp];
+        // a job with the batch's tiles raises the batch's launches; one with
+        // tiles of its own sizes its own (rebuild_tables)
+        const JobTiles bt = batch_tiles(b, q);
+        const bool own = bt.tz != tz || bt.cw != cw || bt.tk != tk || bt.tk2 != tk2;
+        if (!bytes_only && !own)
             for (int pp = 0; pp < P_COUNT; pp++) b->lds[pp] = std::max(b->lds[pp], need_lds[pp]);
     }
 
@@ -1132,20 +1169,20 @@ static bool r1_ok(int nxx, int tk, size_t lim, int variant = 0)
 
 // LDS of each pass for one job (its plans' table lengths and line strides)
 // with the batch's tiles.
-static void job_lds(cq_batch b, const JobD &d, bool q, size_t *out)
+static void job_lds(const JobTiles &tl, const JobD &d, bool q, size_t *out)
 {
     for (int p = 0; p < P_COUNT; p++) out[p] = 0;
     auto xl = [&](int lines) { return lds_job(d.twlx, d.lsx, lines); };
     auto zl = [&](int lines) { return lds_job(d.twlz, d.lsz, lines); };
     if (!q) {
-        out[P_AROW] = xl(b->tz / 2);
-        out[P_ACOL] = zl(b->cw / 2);
+        out[P_AROW] = xl(tl.tz / 2);
+        out[P_ACOL] = zl(tl.cw / 2);
     } else {
-        out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = zl(b->cwq / 2);
-        out[P_QR1] = xl(d.r1_fused ? 3 * b->tk : 2 * b->tk);
-        out[P_LR1] = xl(2 * b->tk);
-        out[P_QR2] = xl(b->tk2);
-        out[P_QR3] = xl(b->tz / 2);
+        out[P_QC1] = out[P_QC2] = out[P_QC3] = out[P_LR2] = zl(tl.cw / 2);
+        out[P_QR1] = xl(d.r1_fused ? 3 * tl.tk : 2 * tl.tk);
+        out[P_LR1] = xl(2 * tl.tk);
+        out[P_QR2] = xl(tl.tk2);
+        out[P_QR3] = xl(tl.tz / 2);
     }
 }
 
@@ -1173,10 +1210,13 @@ struct TileReq {
     bool q;
 };
 
-static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
+struct TileSet {
+    int tz = 16, cw = 16, cwq = 16, tk = 8, tk2 = 8;
+};
+
+static int pick_tiles(const std::vector<TileReq> &req, TileSet &t)
 {
     int iz = 0, ic = 0, icq = 0, ik = 0;   // indices into the candidate lists
-    int ax = 0, az = 0, qx = 0, qz = 0;
     std::vector<int> qxs;
     for (size_t i = 0; i < req.size(); i++) {
         const TileReq &j = req[i];
@@ -1202,31 +1242,120 @@ static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
             while (ik < 3 && krow_bad(kTkCand[ik], kLdsTile)) ik++;
         if (row_bad(kTzCand[iz], kLdsMax) || col_bad(kCwCand[ick], kLdsMax) || (q && krow_bad(kTkCand[ik], kLdsMax)))
             return set_err(CQ_EUNSUPPORTED, "job %d: window %dx%d too large for one workgroup", (int)i, j.nxx, j.nzz);
-        int &mx = q ? qx : ax, &mz = q ? qz : az;
-        mx = std::max(mx, j.nxx);
-        mz = std::max(mz, j.nzz);
     }
-    b->tz = kTzCand[iz];
-    b->cw = kCwCand[ic];
-    b->cwq = kCwCand[icq];
-    b->tk = kTkCand[ik];
+    t.tz = kTzCand[iz];
+    t.cw = kCwCand[ic];
+    t.cwq = kCwCand[icq];
+    t.tk = kTkCand[ik];
     // k_q_r2 takes three or two kz-row tiles at once where every length
     // allows it (more lines per transform keep more of the workgroup busy)
-    b->tk2 = b->tk;
+    t.tk2 = t.tk;
     for (int m : {3, 2}) {
-        const int t2 = m * b->tk;
+        const int t2 = m * t.tk;
         if (t2 > 16) continue;
         bool ok = true;
         for (int n : qxs) ok = ok && has_plan(n, t2 * n) && lds_lines(n, t2) <= kLdsTile;
         if (ok) {
-            b->tk2 = t2;
+            t.tk2 = t2;
             break;
         }
+    }
+    return CQ_OK;
+}
+
+static int choose_tiles(cq_batch b, const std::vector<TileReq> &req)
+{
+    TileSet t;
+    const int rc = pick_tiles(req, t);
+    if (rc) return rc;
+    b->tz = t.tz;
+    b->cw = t.cw;
+    b->cwq = t.cwq;
+    b->tk = t.tk;
+    b->tk2 = t.tk2;
+    int ax = 0, az = 0, qx = 0, qz = 0;
+    std::vector<int> qxs;
+    for (const TileReq &j : req) {
+        int &mx = j.q ? qx : ax, &mz = j.q ? qz : az;
+        mx = std::max(mx, j.nxx);
+        mz = std::max(mz, j.nzz);
+        if (j.q) qxs.push_back(j.nxx);
     }
     b->maxnxx = std::max(ax, qx);
     b->maxnzz = std::max(az, qz);
     set_lds(b, ax, az, qx, qz, qxs);
     return CQ_OK;
+}
+
+// The batch's tiles for a job of its kind.
+static JobTiles batch_tiles(cq_batch b, bool q) { return JobTiles{b->tz, q ? b->cwq : b->cw, b->tk, b->tk2}; }
+
+// Points per workgroup transform of a job's x and z transforms with tiles t:
+// row tiles tz/2 lines, kz-row tiles 2 tk (fused k_q_r1) or tk lines and tk2
+// lines (k_q_r2) of nxx; column tiles cw/2 lines of nzz.
+static void job_needs(const cq_job_desc &j, const JobTiles &t, int variant, bool &fused, int &need_x, int &need_z)
+{
+    const bool q = is_q(j.kind);
+    fused = q && r1_fused_for(j.nxx, t.tk, variant);
+    need_x = std::max((t.tz / 2) * j.nxx, q ? std::max((fused ? 2 : 1) * t.tk, t.tk2) * j.nxx : 0);
+    need_z = (t.cw / 2) * j.nzz;
+}
+
+// True if the job's transforms have plans with the batch's tiles t and no
+// pass needs more LDS than the batch's launches of it have or the tile
+// budget kLdsTile allows, whichever is more.
+static bool tiles_fit(cq_batch b, const cq_job_desc &j, const JobTiles &t)
+{
+    const int variant = b->plan;
+    bool fused;
+    int need_x, need_z;
+    job_needs(j, t, variant, fused, need_x, need_z);
+    CqPlan px, pz;
+    if (!make_plan(j.nxx, need_x, px, variant) || !make_plan(j.nzz, need_z, pz, variant)) return false;
+    JobD d;
+    memset(&d, 0, sizeof d);
+    d.twlx = tw_len(px);
+    d.twlz = tw_len(pz);
+    d.lsx = cq_line_stride(px.n);
+    d.lsz = cq_line_stride(pz.n);
+    d.r1_fused = fused ? 1 : 0;
+    size_t l[P_COUNT];
+    job_lds(t, d, is_q(j.kind), l);
+    for (int p = 0; p < P_COUNT; p++)
+        if (l[p] > std::max(kLdsTile, b->lds[p])) return false;
+    return true;
+}
+
+// A job's tiles: the batch's, or, for a server job whose window those do not
+// fit (tiles_fit: e.g. a Bluestein length, whose inner transform is about
+// twice as long), the tiles a batch of that job alone gets, so that one such
+// job neither is refused nor makes every job's launches take more LDS.
+// Either way a function of the job and the server's tiles: a job's results
+// do not depend on the other jobs.
+static int tiles_for(cq_batch b, const cq_job_desc &j, JobTiles &out)
+{
+    const bool q = is_q(j.kind);
+    out = batch_tiles(b, q);
+    if (!b->own_tiles || tiles_fit(b, j, out)) return CQ_OK;
+    TileSet t;
+    const int rc = pick_tiles({TileReq{j.nxx, j.nzz, q}}, t);
+    if (rc) return rc;
+    out = JobTiles{t.tz, q ? t.cwq : t.cw, t.tk, t.tk2};
+    return CQ_OK;
+}
+
+// The tile width a pass is launched with: z rows (row passes), columns
+// (column passes) or kz rows (k_q_r1 / k_lr_r1: tk, k_q_r2: tk2).
+static int pass_tile(const JobTiles &t, int p)
+{
+    switch (p) {
+    case P_AROW:
+    case P_QR3: return t.tz;
+    case P_QR1:
+    case P_LR1: return t.tk;
+    case P_QR2: return t.tk2;
+    default: return t.cw;
+    }
 }
 
 // Device job table and per-pass work items for the batch's current jobs.
@@ -1266,7 +1395,10 @@ static int rebuild_tables(cq_batch b)
     const int cap = (int)b->djobs_cap;
     std::vector<JobD> hd(2 * (size_t)cap);
     memset(hd.data(), 0, sizeof(JobD) * hd.size());
-    std::vector<int2> items[kGroups][2][P_COUNT];
+    // per role and pass: tile width -> each group's items, and the width's
+    // largest LDS among jobs with tiles of their own
+    std::map<int, std::array<std::vector<int2>, kGroups>> items[2][P_COUNT];
+    std::map<int, size_t> seg_lds[2][P_COUNT];
     b->tmax = 0;
     int nq = 0, na = 0;   // jobs of each kind so far (round-robin over the kind's two groups)
     for (size_t i = 0; i < n; i++) {
@@ -1277,22 +1409,29 @@ static int rebuild_tables(cq_batch b)
         const int g = is_q(h.kind) ? (nq++ & 1) : 2 + (na++ & 1);
         h.grp = g;
         hd[i].dstep = hd[cap + i].dstep = b->dstep + g;
+        const JobTiles &tl = h.tl;
+        const JobTiles bt = batch_tiles(b, is_q(h.kind));
+        const bool own = bt.tz != tl.tz || bt.cw != tl.cw || bt.tk != tl.tk || bt.tk2 != tl.tk2;
         for (int r = 0; r < h.nroles; r++) {
             const int e = r ? cap + (int)i : (int)i;
-            std::vector<int2> *it = items[g][r];
+            // tiles of width w from 0 to n of pass p (origin = tile start)
+            auto add = [&](int p, int w, int n) {
+                std::vector<int2> &v = items[r][p][w][g];
+                for (int o = 0; o < n; o += w) v.push_back(make_int2(e, o));
+                size_t &l = seg_lds[r][p][w];
+                if (own) l = std::max(l, h.lds[p]);
+            };
             if (!is_q(h.kind)) {
-                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_AROW].push_back(make_int2(e, z0));
-                for (int x0 = 0; x0 < h.nxx; x0 += b->cw) it[P_ACOL].push_back(make_int2(e, x0));
+                add(P_AROW, tl.tz, h.nzz);
+                add(P_ACOL, tl.cw, h.nxx);
             } else {
-                for (int x0 = 0; x0 < h.nxx; x0 += b->cwq) {
-                    it[P_QC1].push_back(make_int2(e, x0));
-                    it[h.d[0].lr_rank > 0 ? P_LR2 : P_QC2].push_back(make_int2(e, x0));
-                    it[P_QC3].push_back(make_int2(e, x0));
-                }
                 const bool lr = h.d[0].lr_rank > 0;
-                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk) it[lr ? P_LR1 : P_QR1].push_back(make_int2(e, k0));
-                for (int k0 = 0; k0 < h.d[0].nzh; k0 += b->tk2) it[P_QR2].push_back(make_int2(e, k0));
-                for (int z0 = 0; z0 < h.nzz; z0 += b->tz) it[P_QR3].push_back(make_int2(e, z0));
+                add(P_QC1, tl.cw, h.nxx);
+                add(lr ? P_LR2 : P_QC2, tl.cw, h.nxx);
+                add(P_QC3, tl.cw, h.nxx);
+                add(lr ? P_LR1 : P_QR1, tl.tk, h.d[0].nzh);
+                add(P_QR2, tl.tk2, h.d[0].nzh);
+                add(P_QR3, tl.tz, h.nzz);
             }
         }
     }
@@ -1301,14 +1440,26 @@ static int rebuild_tables(cq_batch b)
     for (int r = 0; r < 2; r++)
         for (int p = 0; p < P_COUNT; p++) {
             b->nitems[r][p] = 0;
-            // groups one after another, each in XCD order
-            for (int g = 0; g < kGroups; g++) {
-                b->goff[r][p][g] = (int)ord[r][p].size();
-                if (items[g][r][p].empty()) continue;
-                const std::vector<int2> o = xcd_order(items[g][r][p]);
-                ord[r][p].insert(ord[r][p].end(), o.begin(), o.end());
+            b->segs[r][p].clear();
+            // segments one after another; in each the groups one after
+            // another, each in XCD order.  A segment of the batch's width
+            // launches with the batch's LDS (raised by its jobs, build_job),
+            // or more if a job with tiles of its own shares the width
+            for (auto &kv : items[r][p]) {
+                cq_batch_s::Seg sg;
+                sg.val = kv.first;
+                sg.lds = seg_lds[r][p][kv.first];
+                if (kv.first == pass_tile(batch_tiles(b, p != P_AROW && p != P_ACOL), p))
+                    sg.lds = std::max(sg.lds, b->lds[p]);
+                for (int g = 0; g < kGroups; g++) {
+                    sg.off[g] = (int)ord[r][p].size();
+                    if (kv.second[g].empty()) continue;
+                    const std::vector<int2> o = xcd_order(kv.second[g]);
+                    ord[r][p].insert(ord[r][p].end(), o.begin(), o.end());
+                }
+                sg.off[kGroups] = (int)ord[r][p].size();
+                b->segs[r][p].push_back(sg);
             }
-            b->goff[r][p][kGroups] = (int)ord[r][p].size();
             if (ord[r][p].empty()) continue;
             if (ord[r][p].size() > b->ditems_cap[r][p]) {
                 if (b->pool) {
@@ -1438,15 +1589,15 @@ static hipError_t raise_lds_limit(int device, const void *kernel, size_t lds)
 }
 
 template <typename K>
-static int launch(cq_batch b, K kernel, int pass, int role, int step)
+static int launch(cq_batch b, K kernel, const cq_batch_s::Seg &sg, int role, int pass, int step)
 {
-    const size_t lds = b->lds[pass];
+    const size_t lds = sg.lds;
     HIPCHK(raise_lds_limit(b->ctx->device, reinterpret_cast<const void *>(kernel), lds));
     if (b->dry) return CQ_OK;
     // one launch per group on its stream, or, unsplit, one for all groups
     for (int g = 0; g < kGroups; g++) {
-        const int o = b->split ? b->goff[role][pass][g] : 0;
-        const int cnt = b->split ? b->goff[role][pass][g + 1] - o : b->nitems[role][pass];
+        const int o = b->split ? sg.off[g] : sg.off[0];
+        const int cnt = (b->split ? sg.off[g + 1] : sg.off[kGroups]) - o;
         if (cnt > 0) {
             hipStream_t st = b->split && g > 0 ? b->side[g] : b->ctx->stream;
             hipLaunchKernelGGL(kernel, dim3(cnt), dim3(CQ_NT), lds, st, b->djobs, b->ditems[role][pass] + o, step);
@@ -1457,11 +1608,11 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
     return CQ_OK;
 }
 
-#define CQ_ARGS pass, role, step
-#define CQ_DISPATCH_TZ(K)                                                          \
-    (b->tz == 16   ? launch(b, K<16>, CQ_ARGS)                                     \
-     : b->tz == 12 ? launch(b, K<12>, CQ_ARGS)                                     \
-     : b->tz == 8  ? launch(b, K<8>, CQ_ARGS) : launch(b, K<4>, CQ_ARGS))
+#define CQ_ARGS sg, role, pass, step
+#define CQ_DISPATCH_TZ(K, T)                                                       \
+    (T == 16   ? launch(b, K<16>, CQ_ARGS)                                         \
+     : T == 12 ? launch(b, K<12>, CQ_ARGS)                                         \
+     : T == 8  ? launch(b, K<8>, CQ_ARGS) : launch(b, K<4>, CQ_ARGS))
 #define CQ_DISPATCH_CW(K, W)                                                       \
     (W == 32   ? launch(b, K<32>, CQ_ARGS)                                         \
      : W == 24 ? launch(b, K<24>, CQ_ARGS)                                         \
@@ -1479,27 +1630,38 @@ static int launch(cq_batch b, K kernel, int pass, int role, int step)
      : T == 3  ? launch(b, K<3>, CQ_ARGS)                                          \
      : T == 2  ? launch(b, K<2>, CQ_ARGS) : launch(b, K<1>, CQ_ARGS))
 
-static int launch_pass(cq_batch b, int pass, int role, int step)
+static int launch_seg(cq_batch b, const cq_batch_s::Seg &sg, int pass, int role, int step)
 {
+    const int v = sg.val;
     switch (pass) {
-    case P_AROW: return CQ_DISPATCH_TZ(k_row_ac);
-    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, b->cw);
-    case P_QC1: return CQ_DISPATCH_CW(k_q_c1, b->cwq);
+    case P_AROW: return CQ_DISPATCH_TZ(k_row_ac, v);
+    case P_ACOL: return CQ_DISPATCH_CW(k_col_ac, v);
+    case P_QC1: return CQ_DISPATCH_CW(k_q_c1, v);
     case P_QR1:
-        return b->tk == 8   ? launch(b, k_q_r1<8>, CQ_ARGS)
-               : b->tk == 4 ? launch(b, k_q_r1<4>, CQ_ARGS)
-               : b->tk == 2 ? launch(b, k_q_r1<2>, CQ_ARGS) : launch(b, k_q_r1<1>, CQ_ARGS);
+        return v == 8   ? launch(b, k_q_r1<8>, CQ_ARGS)
+               : v == 4 ? launch(b, k_q_r1<4>, CQ_ARGS)
+               : v == 2 ? launch(b, k_q_r1<2>, CQ_ARGS) : launch(b, k_q_r1<1>, CQ_ARGS);
     case P_LR1:
-        return b->tk == 8   ? launch(b, k_lr_r1<8>, CQ_ARGS)
-               : b->tk == 4 ? launch(b, k_lr_r1<4>, CQ_ARGS)
-               : b->tk == 2 ? launch(b, k_lr_r1<2>, CQ_ARGS) : launch(b, k_lr_r1<1>, CQ_ARGS);
-    case P_QC2: return CQ_DISPATCH_CW(k_q_c2, b->cwq);
-    case P_LR2: return CQ_DISPATCH_CW(k_lr_c2, b->cwq);
-    case P_QR2: return CQ_DISPATCH_TK(k_q_r2, b->tk2);
-    case P_QC3: return CQ_DISPATCH_CW(k_q_c3, b->cwq);
-    case P_QR3: return CQ_DISPATCH_TZ(k_q_r3);
+        return v == 8   ? launch(b, k_lr_r1<8>, CQ_ARGS)
+               : v == 4 ? launch(b, k_lr_r1<4>, CQ_ARGS)
+               : v == 2 ? launch(b, k_lr_r1<2>, CQ_ARGS) : launch(b, k_lr_r1<1>, CQ_ARGS);
+    case P_QC2: return CQ_DISPATCH_CW(k_q_c2, v);
+    case P_LR2: return CQ_DISPATCH_CW(k_lr_c2, v);
+    case P_QR2: return CQ_DISPATCH_TK(k_q_r2, v);
+    case P_QC3: return CQ_DISPATCH_CW(k_q_c3, v);
+    case P_QR3: return CQ_DISPATCH_TZ(k_q_r3, v);
     }
     return set_err(CQ_EINVAL, "unknown pass %d", pass);
+}
+
+// Every launch segment of a pass (one per tile width among the jobs).
+static int launch_pass(cq_batch b, int pass, int role, int step)
+{
+    for (const cq_batch_s::Seg &sg : b->segs[role][pass]) {
+        const int rc = launch_seg(b, sg, pass, role, step);
+        if (rc) return rc;
+    }
+    return CQ_OK;
 }
 
 // Global step `step` of every wavefield of the batch: one launch per pass
@@ -1932,6 +2094,7 @@ static int server_tiles(cq_batch b, int mx, int mz)
         }
     b->maxnxx = mx;
     b->maxnzz = mz;
+    b->own_tiles = true;
     set_lds(b, mx, mz, mx, mz, {});
     // any job up to mx: fused k_q_r1 within the tile budget, or split
     b->lds[P_QR1] = std::max(std::min(kLdsTile, lds_lines(mx, 3 * b->tk)), lds_lines(mx, 2 * b->tk));

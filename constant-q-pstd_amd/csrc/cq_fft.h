@@ -41,7 +41,7 @@
 // butterflies per thread per stage for radix R
 __host__ __device__ constexpr int cq_nb(int R)
 {
-    return R == 2 ? 8 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : R == 11 ? 2 : R == 15 ? 2 : 1;
+    return R == 2 ? 12 : R == 3 ? 6 : R == 4 ? 5 : R == 5 ? 4 : R == 7 ? 3 : R == 8 ? 3 : R == 9 ? 2 : R == 11 ? 2 : R == 15 ? 2 : 1;
 }
 
 // A transform plan: stages of radices rad[0 .. ns) for n points.  A length

@@ -132,7 +132,9 @@ int cq_batch_destroy(cq_batch b);
  * until the device memory budget and the job limit admit them, then join
  * the launches of the jobs already running at the next global step.
  * Finished jobs leave the batch, their results stay on the host until
- * released. */
+ * released.  Workgroup tiles are chosen for the largest window; a job whose
+ * transforms those do not fit within the LDS budget (e.g. a length with a
+ * large prime factor) runs with tiles of its own in launches of its own. */
 typedef struct cq_server_s *cq_server;
 
 typedef struct {

@@ -14,7 +14,7 @@ LIB = C.CDLL(os.path.join(ROOT, "constant-q-pstd_amd", "libcq_hip.so"))
 LIB.cq_fft_plan.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
 
 NT = 512                                   # threads per workgroup
-NB = {2: 8, 3: 6, 4: 5, 5: 4, 7: 3, 8: 3, 9: 2, 11: 2, 15: 2}   # butterflies per thread, else 1
+NB = {2: 12, 3: 6, 4: 5, 5: 4, 7: 3, 8: 3, 9: 2, 11: 2, 15: 2}   # butterflies per thread, else 1
 PRIMES = (2, 3, 5, 7, 11, 13, 17)
 RADICES = {2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17, 15, 21, 25, 27}
 COMPOSITES = {15, 21, 25, 27}

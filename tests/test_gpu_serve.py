@@ -37,8 +37,8 @@ def same(a, b):
     return np.array_equal(a, b)
 
 
-def run_alone(ctx, job):
-    s = ctx.server(MAXX, MAXZ)
+def run_alone(ctx, job, mx=MAXX, mz=MAXZ):
+    s = ctx.server(mx, mz)
     i = s.submit(job)
     while s.state(i) != engine.DONE:
         s.advance(64)
@@ -87,6 +87,33 @@ def test_serve_job_limit_runs_one_at_a_time(ctx):
     for k, i in enumerate(ids):
         assert same(s.result(i), run_alone(ctx, js[k])), k
     s.close()
+
+
+def test_serve_window_with_tiles_of_its_own(ctx):
+    """Windows 409 deep (a prime: Bluestein, 840-point inner transforms) on a
+    server for windows up to 64 x 600, whose 24-column tiles would need 97 KB
+    of LDS for them (12 inner lines): they run with tiles of their own,
+    launched beside the server's, next to jobs on the server's tiles.  Every
+    result equals the job alone on a server of the same limits, and the
+    gathers match the FP64 oracle."""
+    mx, mz = 64, 600
+    jobs = [synth.small_job(nxt=40, nzt=399, pml=5, nt=40, kind=k, seed=61 + k) for k in (0, 1)]
+    jobs += [synth.small_job(nxt=40, nzt=24, pml=5, nt=60, kind=0, seed=63),
+             synth.small_job(nxt=41, nzt=25, pml=7, nt=50, kind=1, seed=64)]
+    for j in jobs:
+        print("case window", j["nxx"], j["nzz"], "kind", j["kind"], flush=True)
+    alone = [run_alone(ctx, j, mx, mz) for j in jobs]
+    s = ctx.server(mx, mz)
+    ids = [s.submit(j) for j in jobs]
+    while any(s.state(i) != engine.DONE for i in ids):
+        s.advance(16)
+    for k, i in enumerate(ids):
+        assert same(s.result(i), alone[k]), k
+    s.close()
+    for k in (0, 1):
+        e = engine.rel_l2(alone[k], orc.forward(jobs[k]))
+        print("kind", k, "rel-L2 vs FP64", e)
+        assert e <= TOL
 
 
 def test_serve_rejects_oversized_and_unfinished(ctx):
