@@ -842,8 +842,8 @@ static int build_job(cq_batch b, const cq_job_desc &j, JobH &h, size_t *bytes_on
     base.lsx = cq_line_stride(base.px.n);
     base.lsz = cq_line_stride(base.pz.n);
     {
-        // the passes' LDS for this window with the batch's tiles; a
-        // Bluestein length may need more than the batch's widest window
+        // the passes' LDS for this window with its tiles; a Bluestein
+        // length may need more than the batch's widest window
         size_t need_lds[P_COUNT];
         job_lds(h.tl, base, q, need_lds);
         for (int pp = 0; pp < P_COUNT; pp++)
@@ -1168,7 +1168,7 @@ static bool r1_ok(int nxx, int tk, size_t lim, int variant = 0)
 }
 
 // LDS of each pass for one job (its plans' table lengths and line strides)
-// with the batch's tiles.
+// with tiles tl.
 static void job_lds(const JobTiles &tl, const JobD &d, bool q, size_t *out)
 {
     for (int p = 0; p < P_COUNT; p++) out[p] = 0;
