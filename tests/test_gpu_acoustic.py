@@ -119,3 +119,32 @@ def test_bad_inputs_raise(ctx):
     bad["nt"] = 0
     with pytest.raises(engine.CqError):
         ctx.batch([bad])
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_no_receivers(ctx, kind):
+    """A shot with no receivers (nr = 0, an empty gather): it runs alone and
+    beside shots that record, its gather is empty, and its final pressure
+    field equals the same shot's with receivers bit for bit and the FP64
+    oracle's within the modeling tolerance."""
+    job = synth.small_job(nxt=40, nzt=24, pml=5, nt=60, kind=kind, seed=71)
+    bare = dict(job)
+    bare["gxg"] = np.zeros(0, np.int32)
+    bare["gzg"] = np.zeros(0, np.int32)
+    other = synth.small_job(nxt=41, nzt=25, pml=6, nt=50, kind=1 - kind, seed=72)
+    _, st = orc.forward(bare, want_state=True)
+    b = ctx.batch([bare, other, job])
+    b.run()
+    assert b.rec(0).shape == (60, 0)
+    p0 = b.p(0)
+    assert np.array_equal(p0, b.p(2))
+    e = engine.rel_l2(p0, st["p"])
+    print("kind", kind, "rel-L2 final p", e)
+    assert e <= TOL
+    b.close()
+    s = ctx.server(64, 48)
+    i = s.submit(bare)
+    while s.state(i) != engine.DONE:
+        s.advance(32)
+    assert s.result(i).size == 0
+    s.close()

@@ -150,3 +150,24 @@ def test_mixed_batch_in_uneven_chunks_equals_run(ctx):
     for i in range(2):
         assert np.array_equal(got[i][0], ref[i][0]) and np.array_equal(got[i][1], ref[i][1]), i
     assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_rtm_no_receivers(ctx, kind):
+    """An RTM shot with no receivers (nr = 0: no observed data): the receiver
+    wavefield stays zero, so the image is zero, and the illumination equals
+    the same shot's with receivers bit for bit (it depends on the source
+    wavefield alone)."""
+    job = rtm_job(kind, nt=60, snaps=5, seed=73)
+    bare = dict(job)
+    bare["gxg"] = np.zeros(0, np.int32)
+    bare["gzg"] = np.zeros(0, np.int32)
+    bare["obs"] = np.zeros((60, 0), np.float32)
+    b = ctx.batch([bare, job])
+    b.run()
+    img0, ssg0 = b.image(0)
+    img1, ssg1 = b.image(1)
+    b.close()
+    assert not img0.any()
+    assert ssg0.any() and np.array_equal(ssg0, ssg1)
+    assert img1.any()
