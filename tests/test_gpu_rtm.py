@@ -171,3 +171,30 @@ def test_rtm_no_receivers(ctx, kind):
     assert not img0.any()
     assert ssg0.any() and np.array_equal(ssg0, ssg1)
     assert img1.any()
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("nt", [1, 2, 3])
+def test_shortest_time_axes_vs_oracle(ctx, kind, nt):
+    """Time axes of one to three samples (a one-step revolve schedule has no
+    forward sweep: its first turn is the receiver step of sample 0) against
+    the FP64 oracle; an all-zero reference must be matched by zeros."""
+    if kind < 2:
+        job = synth.small_job(nxt=40, nzt=24, pml=5, nt=nt, kind=kind, seed=74)
+        ref = orc.forward(job)
+    else:
+        job = rtm_job(kind, nt=nt, snaps=2, seed=75)
+        ref = orc.rtm(job, host.revolve_trace(nt, 2)[0])
+    print("case kind", kind, "nt", nt, flush=True)
+    b = ctx.batch([job])
+    b.run()
+    got = b.rec(0) if kind < 2 else b.image(0)
+    b.close()
+    checks = [(got, ref, 1e-4)] if kind < 2 else [(got[0], ref[0], TOL_IMG), (got[1], ref[1], TOL_SSG)]
+    for g, r, tol in checks:
+        if not np.any(r):
+            assert not np.any(g)
+            continue
+        e = engine.rel_l2(g, r)
+        print("rel-L2", e)
+        assert e <= tol
