@@ -148,3 +148,24 @@ def test_no_receivers(ctx, kind):
         s.advance(32)
     assert s.result(i).size == 0
     s.close()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("nxt,nzt,corner", [(4, 4, False), (5, 6, False), (7, 9, True), (13, 12, True)])
+def test_smallest_windows_vs_oracle(ctx, kind, nxt, nzt, corner):
+    """The smallest windows the engine accepts (8 x 8 padded with a 2-point
+    CPML) up to 17 x 16: single-stage transforms of 8, 9, 11, 13, 16 and 17
+    points, tiles wider and deeper than the window; with `corner` the source
+    sits in the CPML corner cell (0, 0)."""
+    job = synth.small_job(nxt=nxt, nzt=nzt, pml=2, nt=30, kind=kind, seed=76)
+    if corner:
+        job["xsg"], job["zsg"] = 0, 0
+    print("case kind", kind, "window", job["nxx"], job["nzz"], "corner", corner, flush=True)
+    ref, st = orc.forward(job, want_state=True)
+    b = ctx.batch([job])
+    b.run()
+    e = engine.rel_l2(b.rec(0), ref)
+    ep = engine.rel_l2(b.p(0), st["p"])
+    b.close()
+    print("rel-L2 gather", e, "final p", ep)
+    assert e <= TOL and ep <= TOL
