@@ -46,7 +46,7 @@ def test_taylor_decomposition_reduces_to_taylor_path(ctx):
     assert e_t <= 1e-5 and e_o <= 1e-4
 
 
-@pytest.mark.parametrize("rank", [2, 4, 6])
+@pytest.mark.parametrize("rank", [1, 2, 4, 6, 8])
 def test_fraclap_lowrank_vs_oracle_small(ctx, rank):
     job = synth.small_job(nxt=60, nzt=35, pml=8, nt=150, kind=1, seed=82)
     lrjob, lr = synth.with_lowrank(job, rank, seed=5)
