@@ -39,7 +39,7 @@ def rtm_job(kind, nxt=40, nzt=24, pml=5, nt=120, snaps=6, seed=7, eta=1e-20):
 
 
 @pytest.mark.parametrize("kind", [2, 3])
-@pytest.mark.parametrize("nt,snaps", [(120, 6), (90, 40), (64, 1)])
+@pytest.mark.parametrize("nt,snaps", [(120, 6), (90, 40), (64, 1), (120, 64)])
 def test_rtm_vs_oracle(ctx, kind, nt, snaps):
     job = rtm_job(kind, nt=nt, snaps=snaps)
     print("case kind", kind, "nt", nt, "snaps", snaps, job["nxx"], job["nzz"], flush=True)
