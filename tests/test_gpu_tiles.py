@@ -121,6 +121,26 @@ def test_benchmark_window_radix8_plan(ctx, kind):
     assert e <= TOL
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_compressed_twiddles_short_columns(ctx, kind):
+    """1080-point columns (8 15 9) in 16-column tiles: eight lines and the
+    full 1080-entry twiddle table would take 86.4 KB of LDS, so the plan keeps
+    the last stage's first powers only (239 entries, 79.7 KB), as it
+    otherwise does only above 2048 points.  Against the FP64 oracle."""
+    job = synth.small_job(nxt=40, nzt=1000, pml=40, nt=40, kind=kind, seed=77)
+    assert job["nzz"] == 1080
+    print("case window", job["nxx"], job["nzz"], "kind", kind, flush=True)
+    t = {"cw": 16, "cwq": 16}
+    b = ctx.batch([job], tiles=t)
+    assert all(b.tiles()[k] == v for k, v in t.items())
+    b.run()
+    got = b.rec(0)
+    b.close()
+    e = engine.rel_l2(got, orc.forward(job))
+    print("rel-L2 gather", e)
+    assert e <= TOL
+
+
 def test_autotune_and_cache(ctx, mix, tmp_path):
     jobs, refs = mix
     cache = tmp_path / "tiles.txt"
