@@ -16,7 +16,7 @@ pass once over every shot) with its measured traffic,
 the FP32 reference-faithful CPU path (the oracle's FP32 build: 9 full 2-D C2C
 FFTs per step, OpenMP) timed on a bounded sample of the same workload, the
 serving phase (continuous batching of Poisson arrivals: throughput, p50/p99
-latency; shortest remaining work first within a run set of 32 M work cells,
+latency; shortest remaining work first within a run set of 20 M work cells,
 and beside it every resident job advancing at every step) and the Q-RTM
 phase (32 revolve-checkpointed shots).  Cell-steps
 count the windows the shots ask for (CPML included); the engine computes on
@@ -336,7 +336,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-window", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=50)
-    ap.add_argument("--serve-run-cells", type=int, default=32000000,
+    ap.add_argument("--serve-run-cells", type=int, default=20000000,
                     help="server run set in work cells: the jobs with the least remaining work "
                          "advance (0: every resident job advances every step)")
     ap.add_argument("--no-serve-all", action="store_true",
