@@ -206,14 +206,14 @@ def profiled_traffic(kernel):
     return None
 
 
-def serve_bench(rate, window_s, seed, chunk, run_cells=0):
+def serve_bench(rate, window_s, seed, chunk, run_cells=0, nx_max=4096, nz_max=1024):
     """Serving workload (Poisson arrivals of mixed acoustic / viscoacoustic /
     Q-RTM requests, each with its own window, nt, dx and wavelet) through the
     engine's continuous-batching server.  Requests are generated before the
     clock starts and submitted at their arrival times; the server admits them
     into the running launches.  Latency = completion - arrival, wall clock."""
     from cqpy import engine, host, synth
-    reqs = synth.serve_jobs(rate=rate, window_s=window_s, seed=seed)
+    reqs = synth.serve_jobs(rate=rate, window_s=window_s, seed=seed, nx_range=(512, nx_max), nz_range=(256, nz_max))
     mx = max(j["nxx"] for _, j, _ in reqs)
     mz = max(j["nzz"] for _, j, _ in reqs)
     useful = executed = 0
@@ -262,7 +262,7 @@ def serve_bench(rate, window_s, seed, chunk, run_cells=0):
                         "viscoacoustic, 20%% Q-RTM with 40 checkpoints), windows %d-%d x %d-%d "
                         "cells + 40-pt CPML (padded to 7-smooth sizes, at most %dx%d), nt 2000-8000, "
                         "dx 5-12.5 m, Ricker 10-30 Hz"
-                        % (rate, window_s, len(reqs), 512, 4096, 256, 1024, mx, mz),
+                        % (rate, window_s, len(reqs), 512, nx_max, 256, nz_max, mx, mz),
             "jobs": len(reqs), "wall_s": round(wall, 3),
             "useful_mcell_steps_per_s": round(useful / (wall - first) / 1e6, 1),
             "executed_mcell_steps_per_s": round(executed / (wall - first) / 1e6, 1),
@@ -336,6 +336,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-window", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-nx-max", type=int, default=4096,
+                    help="widest serving window (interior columns; 1216 with --serve-nz-max 600 gives the "
+                         "smaller-window mix of earlier measurements)")
+    ap.add_argument("--serve-nz-max", type=int, default=1024, help="deepest serving window (interior rows)")
     ap.add_argument("--serve-run-cells", type=int, default=20000000,
                     help="server run set in work cells: the jobs with the least remaining work "
                          "advance (0: every resident job advances every step)")
@@ -442,13 +446,15 @@ def main():
     serve = None
     if not args.no_serve:
         print("serving phase", file=sys.stderr, flush=True)
-        serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, args.serve_run_cells)
+        serve = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, args.serve_run_cells,
+                            args.serve_nx_max, args.serve_nz_max)
         print("serve:", json.dumps(serve), file=sys.stderr, flush=True)
         if args.serve_run_cells > 0 and not args.no_serve_all:
             # the same arrivals with every resident job advancing at every
             # global step (throughput-first policy), for comparison
             print("serving phase, all resident jobs advancing", file=sys.stderr, flush=True)
-            serve["all_resident"] = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, 0)
+            serve["all_resident"] = serve_bench(args.serve_rate, args.serve_window, 501, args.serve_chunk, 0,
+                                                args.serve_nx_max, args.serve_nz_max)
             print("serve all resident:", json.dumps(serve["all_resident"]), file=sys.stderr, flush=True)
 
     qrtm = None
