@@ -24,7 +24,9 @@ windows widened to FFT-friendly sizes (about 1% more cells in the batched
 workload), reported beside as computed_mcell_steps_per_s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch128|config1]
-                       [--no-serve] [--no-qrtm] [--no-plain] [--no-cpu]
+                       [--no-serve] [--no-qrtm] [--no-plain] [--no-cpu] [--no-accuracy]
+                       [--serve-run-cells C] [--no-serve-all] [--serve-nx-max NX] [--serve-nz-max NZ]
+                       [--graph-steps G] [--copy-calibration GB]
 """
 import argparse
 import json
