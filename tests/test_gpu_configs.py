@@ -135,3 +135,27 @@ def test_serve_largest_window_alone_equals_mix(ctx):
         else:
             assert np.isfinite(a).all() and np.abs(a).max() > 0
             assert np.array_equal(a, m), k
+
+
+def test_batch128_full_workload_linear_in_the_wavelet(ctx):
+    """The batched workload at its full size (128 shots, half acoustic and
+    half viscoacoustic, 1296 x 680 interior windows) through a property that
+    holds exactly: the step is linear in the wavefield and doubling is exact
+    in floating point, so doubling the wavelet doubles every gather bit for
+    bit (100 steps, every shot)."""
+    kinds = [i % 2 for i in range(128)]
+    jobs = synth.batch128_jobs(nt=100, nshots=128, kinds=kinds)
+    twice = []
+    for j in jobs:
+        d = dict(j)
+        d["rik"] = (2.0 * np.asarray(j["rik"], np.float32)).astype(np.float32)
+        twice.append(d)
+    out = []
+    for js in (jobs, twice):
+        b = ctx.batch(js)
+        b.run()
+        out.append([b.rec(i) for i in range(len(js))])
+        b.close()
+    for i in range(len(jobs)):
+        assert np.any(out[0][i]), i
+        assert np.array_equal(out[1][i], 2.0 * out[0][i]), i
