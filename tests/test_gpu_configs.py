@@ -159,3 +159,28 @@ def test_batch128_full_workload_linear_in_the_wavelet(ctx):
     for i in range(len(jobs)):
         assert np.any(out[0][i]), i
         assert np.array_equal(out[1][i], 2.0 * out[0][i]), i
+
+
+def test_batch128_full_workload_shift_invariant_in_time(ctx):
+    """The batched workload at its full size: the step does not depend on the
+    time index except through the wavelet sample, so a wavelet delayed by k
+    steps delays every gather by k rows bit for bit (zeros before)."""
+    k, nt = 7, 80
+    kinds = [i % 2 for i in range(128)]
+    jobs = synth.batch128_jobs(nt=nt, nshots=128, kinds=kinds)
+    late = []
+    for j in jobs:
+        d = dict(j)
+        r = np.zeros(nt, np.float32)
+        r[k:] = np.asarray(j["rik"], np.float32)[:nt - k]
+        d["rik"] = r
+        late.append(d)
+    out = []
+    for js in (jobs, late):
+        b = ctx.batch(js)
+        b.run()
+        out.append([b.rec(i) for i in range(len(js))])
+        b.close()
+    for i in range(len(jobs)):
+        assert not np.any(out[1][i][:k]), i
+        assert np.array_equal(out[1][i][k:], out[0][i][:nt - k]), i
