@@ -151,7 +151,9 @@ def plain_bench(ctx, pair, steps):
     """The plain path (cq_plain_forward: one shot, one launch sequence per
     step, the gather row copied to the host and the stream synchronised
     after every step) on one acoustic and one viscoacoustic shot of the
-    workload; the mix rate weights them like the batch (equal cell-steps)."""
+    workload, each a whole shot of `steps` steps including its allocation
+    and upload as the reference's acforward does; the mix rate weights them
+    like the batch (equal cell-steps)."""
     res, cells, secs = {}, 0.0, 0.0
     for job in pair:
         pj = shortened(job, steps)
@@ -328,7 +330,9 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=16,
                     help="global steps per captured HIP graph (0: direct launches)")
     ap.add_argument("--accuracy-steps", type=int, default=24)
-    ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="steps of the plain path per shot (0: the workload's whole time axis, so that its "
+                         "per-shot allocation and upload weigh as in a real shot)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a streaming copy of this many GB (counter calibration)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) phase")
@@ -437,7 +441,9 @@ def main():
             pair.append(cand[0])
     plain = None
     if not args.no_plain:
-        plain = plain_bench(ctx, pair, args.plain_steps)
+        plain_steps = args.plain_steps or int(pair[0]["nt"])
+        plain = plain_bench(ctx, pair, plain_steps)
+        plain["steps_per_shot"] = plain_steps
         print("plain:", json.dumps(plain), file=sys.stderr, flush=True)
     accuracy = None
     if not args.no_accuracy:
