@@ -169,3 +169,22 @@ def test_smallest_windows_vs_oracle(ctx, kind, nxt, nzt, corner):
     b.close()
     print("rel-L2 gather", e, "final p", ep)
     assert e <= TOL and ep <= TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_widest_window_vs_oracle(ctx, kind):
+    """A 5103-point-wide window (27 27 7): two lines of a 4-depth row tile
+    and the compressed twiddle table take 93 KB, more than the 80 KB tile
+    budget, so the row passes launch with up to 160 KB of LDS (one workgroup
+    per CU).  Against the FP64 oracle."""
+    job = synth.small_job(nxt=5086, nzt=32, pml=8, nt=20, kind=kind, seed=78)
+    assert job["nxx"] == 5103
+    print("case window", job["nxx"], job["nzz"], "kind", kind, flush=True)
+    ref = orc.forward(job)
+    b = ctx.batch([job])
+    print("tiles", b.tiles(), flush=True)
+    b.run()
+    e = engine.rel_l2(b.rec(0), ref)
+    b.close()
+    print("rel-L2 gather", e)
+    assert e <= TOL
